@@ -1,0 +1,202 @@
+/*
+ * parent_hip.h -- C ABI of the MI355X (gfx950) all-pairs entropy / mutual
+ * information engine.
+ *
+ * The reference (PARENT, markusfleck/PARENT) has no library interface for this
+ * path: PARENT.x and get_PAR_MIST.x are monolithic MPI programs behind a
+ * process + file boundary (.bat in, .par / MIST text out).  Each entry point
+ * below replaces one block of those programs or of their IO library; the
+ * comment above it names the reference code it stands for.
+ *
+ * Conventions
+ *   - every function returns int: 0 = PARENT_OK, nonzero = a PARENT_E_* code;
+ *     parent_engine_error() / parent_last_error() give the message.
+ *   - caller-owned buffers, plain pointers and sizes; no exceptions cross
+ *     the boundary.
+ *   - one engine owns one HIP stream and its device allocations on one GPU;
+ *     an engine is not re-entrant (one thread at a time).
+ *   - global dof id: bonds 0..nB-1, angles nB..nB+nA-1, dihedrals
+ *     nB+nA..D-1 with nB = nD+2, nA = nD+1, D = 3nD+3 (PARENT.cpp:298-299).
+ *   - "frame-major values": value[f*D + g] for frame f and global dof g.
+ *   - bins[6] = {bonds1D, angles1D, dihedrals1D, bonds2D, angles2D,
+ *     dihedrals2D}, the six counts of PARENT.x's command line
+ *     (PARENT.cpp:215-238) and of the .par header (io_binary.cpp:385-396).
+ *   - 2D results ("s2d") are D(D-1)/2 doubles in .par body order: the
+ *     bonds-bonds half matrix in reverse order, bonds-angles, bonds-dihedrals,
+ *     angles-angles (reverse), angles-dihedrals, dihedrals-dihedrals
+ *     (reverse) (io_binary.cpp:606-631, PARENT.cpp:1107-1112).
+ */
+#ifndef PARENT_HIP_H
+#define PARENT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    PARENT_OK = 0,
+    PARENT_E_INVALID = 1,     /* bad argument or corrupt header (reference: return 1 / MPI_Abort(..,1)) */
+    PARENT_E_NOMEM = 2,       /* host or device allocation failed (read_PAR_body returns 2) */
+    PARENT_E_IO = 3,          /* file cannot be opened, read or written */
+    PARENT_E_DATA = 4,        /* a bond length < 0 (PARENT.cpp:522-525) */
+    PARENT_E_RANGE = 5,       /* a value falls outside its histogram (reference: out-of-bounds write) */
+    PARENT_E_DEVICE = 6,      /* HIP runtime error */
+    PARENT_E_UNSUPPORTED = 7  /* request outside what this build implements (e.g. > 255 bins) */
+};
+
+/* pair-kernel selection for parent_engine_pairs() */
+enum {
+    PARENT_METHOD_PLAIN = 0,  /* one dof pair per workgroup, LDS atomic histogram, per-cell FP64 epilogue */
+    PARENT_METHOD_ATOMIC = 1, /* tiled: many pairs per workgroup, LDS atomic histograms, fused n*ln(n) epilogue */
+    PARENT_METHOD_MFMA = 2,   /* one-hot int8 x int8 -> int32 MFMA contraction, fused epilogue */
+    PARENT_METHOD_AUTO = 3    /* autotuned choice per bin count */
+};
+
+/* MIST totals, in the order get_PAR_MIST.x prints them (get_PAR_MIST.cpp:754-765) */
+enum {
+    PARENT_TOT_S1D_B = 0, PARENT_TOT_S1D_A, PARENT_TOT_S1D_D,
+    PARENT_TOT_MI_BB, PARENT_TOT_MI_BA, PARENT_TOT_MI_BD,
+    PARENT_TOT_MI_AA, PARENT_TOT_MI_AD, PARENT_TOT_MI_DD,
+    PARENT_TOT_TOTAL, PARENT_TOT_COUNT
+};
+
+/* ------------------------------------------------------------------ */
+/* .bat / .par files (host)                                            */
+/* ------------------------------------------------------------------ */
+
+typedef struct parent_bat parent_bat;
+
+/* Opens a .bat and reads its header.  Replaces read_BAT_header
+ * (io_binary.cpp:107-200), with the same validation: version >= 0,
+ * double_prec in {0,1}, nDihedrals >= 0, numFrames >= 1. */
+int parent_bat_open(const char* path, parent_bat** out);
+
+/* Header fields of an opened .bat. */
+int parent_bat_info(const parent_bat* bat, int* n_dihedrals, int* n_frames, int* double_prec);
+
+/* Reads all frames into frame-major values (float if double_prec == 0,
+ * double otherwise), D values per frame in global dof order.  Replaces the
+ * read_BAT_frame loop and the per-frame scatter (io_binary.cpp:281-368,
+ * PARENT.cpp:443-472).  Fails with PARENT_E_IO on a truncated file. */
+int parent_bat_read_frames(parent_bat* bat, void* values);
+
+void parent_bat_close(parent_bat* bat);
+
+/* Writes a .par v4: header copied from the .bat's (write_PAR_header,
+ * io_binary.cpp:371-442) then the body (write_PAR_body, io_binary.cpp:606-631):
+ * s1d[D] then s2d[D(D-1)/2]. */
+int parent_par_write(const char* path, const parent_bat* header, const int bins[6],
+                     const double* s1d, const double* s2d);
+
+/* Reads a .par: header fields and body arrays.  Replaces read_PAR_header +
+ * read_PAR_body (io_binary.cpp:446-673), with the same validation.  s1d/s2d
+ * may be NULL to query sizes; otherwise they hold D and D(D-1)/2 doubles. */
+int parent_par_read(const char* path, int* n_dihedrals, int* n_frames, int bins[6],
+                    double* s1d, double* s2d);
+
+/* Message of the last failed parent_bat_* / parent_par_* / parent_synth_*
+ * call on this thread. */
+const char* parent_last_error(void);
+
+/* ------------------------------------------------------------------ */
+/* engine (device)                                                     */
+/* ------------------------------------------------------------------ */
+
+typedef struct parent_engine parent_engine;
+
+typedef struct {
+    double ms_upload;     /* H2D of the trajectory (0 if it was already on the device) */
+    double ms_prepare;    /* ranges + modfit + pre-binning + 1D entropies */
+    double ms_pairs;      /* all 2D histograms + entropy epilogue */
+    double ms_mist;       /* device Prim */
+    double pair_updates;  /* sum over pairs of frames */
+    int method;           /* PARENT_METHOD_* actually used by the last parent_engine_pairs */
+    int kernel_variant;   /* tile configuration id chosen for that method */
+} parent_stats;
+
+/* Creates an engine on HIP device `device`.  No counterpart in the
+ * reference (MPI_Init + buffers, PARENT.cpp:131-141, 300-440). */
+int parent_engine_create(int device, parent_engine** out);
+void parent_engine_destroy(parent_engine* eng);
+const char* parent_engine_error(const parent_engine* eng);
+
+/* Hands the engine a trajectory.  values: frame-major, float when
+ * double_prec == 0 else double.  on_device == 0: host pointer, copied to the
+ * device now; on_device == 1: device pointer the engine borrows until the
+ * next set_trajectory/destroy.  Replaces the read-in + scatter of
+ * PARENT.cpp:443-493 (the data then lives in HBM, not per-rank RAM). */
+int parent_engine_set_trajectory(parent_engine* eng, int n_dihedrals, int n_frames,
+                                 int double_prec, const void* values, int on_device);
+
+/* Extrema, dihedral modfit, bin indices and 1D entropies: PARENT.cpp:503-772
+ * (rules at 511-637, 1D at 644-772).  Bins must be in [1,255]. */
+int parent_engine_prepare(parent_engine* eng, const int bins[6]);
+
+/* All 2D entropies S_ij for the bins of the last prepare:
+ * PARENT.cpp:774-1479 (histogram fill at 802-804 and its 14 twins, epilogues
+ * at 805-822, 838-855, 870-885, 1158-1175, 1190-1205, 1400-1413). */
+int parent_engine_pairs(parent_engine* eng, int method);
+
+/* Copies results to the host (synchronises the engine's stream). */
+int parent_engine_read_s1d(parent_engine* eng, double* s1d /* D */);
+int parent_engine_read_s2d(parent_engine* eng, double* s2d /* D(D-1)/2 */);
+
+/* Per-dof ranges after prepare: min, max (FP64, PARENT.cpp:528-636) and the
+ * dihedral shift modFit (PARENT.cpp:609; 0 for bonds/angles). */
+int parent_engine_read_ranges(parent_engine* eng, double* vmin, double* vmax, double* modfit);
+
+/* The 2D bin indices, dof-major: out[g*n_frames + f] (PARENT.cpp:803 index
+ * expressions, here computed once per dof and frame). */
+int parent_engine_read_bins(parent_engine* eng, uint8_t* out);
+
+/* Histogram counts of listed pairs through `method`'s fill path, without the
+ * epilogue: pairs[2k], pairs[2k+1] = (gi < gj); counts of pair k are
+ * bins(gi) x bins(gj) int32, row = gi, concatenated.  For parity checks of
+ * the long long histo[][] of PARENT.cpp:790-804. */
+int parent_engine_pair_counts(parent_engine* eng, int method, int n_pairs, const int32_t* pairs,
+                              int32_t* counts);
+
+/* MIST on the engine's current S1D/S2D (get_PAR_MIST.cpp:396-766), as an
+ * O(V^2) device Prim that reproduces the reference's single-thread scan order
+ * and tie rule.  edges: (D-1) x 4 int32 {type_in, type_out, index_in,
+ * index_out} (0-based within type), mi: D-1 doubles, left: D-1 int32 (the
+ * "left to process" count printed per line), totals: PARENT_TOT_COUNT
+ * doubles. */
+int parent_engine_mist(parent_engine* eng, int32_t* edges, double* mi, int32_t* left,
+                       double* totals);
+
+/* Loads S1D/S2D from host arrays (e.g. a .par read with parent_par_read) so
+ * parent_engine_mist can run without a trajectory (the get_PAR_MIST.x path,
+ * get_PAR_MIST.cpp:211-327). */
+int parent_engine_set_entropies(parent_engine* eng, int n_dihedrals, const double* s1d,
+                                const double* s2d);
+
+/* Waits for the engine's stream. */
+int parent_engine_sync(parent_engine* eng);
+
+/* Phase timings of the last calls (HIP events on the engine's stream);
+ * replaces the gettimeofday stopwatches of PARENT.cpp:95-97,498,1489,1501-1505. */
+int parent_engine_stats(parent_engine* eng, parent_stats* out);
+
+/* ------------------------------------------------------------------ */
+/* synthetic trajectories (no counterpart in the reference: test and   */
+/* benchmark input generator)                                          */
+/* ------------------------------------------------------------------ */
+
+/* Fills frame-major float values [n_frames][3*atoms-6] of a linear chain of
+ * `atoms` atoms: Gaussian bonds, wrapped-normal angles, von Mises mixture
+ * dihedrals in [0, 2*pi), 20% of dofs sharing one of 10 latent factors.
+ * Deterministic in (atoms, n_frames, seed) for any thread count. */
+int parent_synth_values(int atoms, int n_frames, uint64_t seed, float* values, int nthreads);
+
+/* Writes the same trajectory as a single-precision .bat v3 file. */
+int parent_synth_write_bat(const char* path, int atoms, int n_frames, uint64_t seed, int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PARENT_HIP_H */

@@ -1,0 +1,72 @@
+// Kernel launch interfaces between the engine (host) and the HIP translation
+// units.  Internal; not part of the C ABI.
+#pragma once
+
+#include "kernels_common.h"
+
+namespace parent {
+
+struct PrepareArgs {
+    const void* values;  // frame-major [N][D]
+    int double_prec, nD, D, N, Np;
+    double pi;
+    Bins3 B1, B2;
+    unsigned long long *emin, *emax;  // [D] scratch
+    unsigned* mfhist;                 // [nD][100] scratch
+    int* flags;
+    double *vmin, *vmax, *modfit;     // [D]
+    uint8_t* bins2;                   // [D][Np]
+    uint8_t* bins1;                   // [D][Np] or nullptr when 1D bins == 2D bins for all types
+    double *s1d, *c2;                 // [D]
+    double* jtab;                     // [D][256]
+    double* nlogn;                    // [N+1]
+};
+hipError_t launch_prepare(const PrepareArgs& a, hipStream_t s);
+
+// Common inputs of every pair kernel.
+struct PairInputs {
+    const uint8_t* bins;  // dof-major [D][Np]
+    int nD, D, N, Np;
+    Bins3 B;              // 2D bins per type
+    const double *vmin, *vmax, *jtab, *c2, *nlogn;
+    double logN;
+    double* s2d;          // [D(D-1)/2] .par order
+};
+
+// Optional histogram dump of selected pairs (parity checks).
+struct CountsDump {
+    int n;                      // number of requested pairs
+    const int* pairs;           // [n][2]
+    const long long* offsets;   // [n] offset of each pair's counts in `out`
+    int* out;
+};
+
+// plain: one pair per workgroup
+hipError_t launch_pairs_plain(const PairInputs& in, const CountsDump* dump, hipStream_t s);
+
+// tiled LDS-atomic kernel
+struct AtomicConfig {
+    int tr, tc;    // row / column dofs per tile
+    int block;     // threads per workgroup
+};
+int atomic_config_count();
+AtomicConfig atomic_config(int id);
+size_t atomic_lds_bytes(const AtomicConfig& c, int bmax);
+hipError_t launch_pairs_atomic(const PairInputs& in, int config_id, int bmax, const int* tiles,
+                               int ntiles, const int* dump_begin, const int* dump_entries,
+                               int* dump_out, hipStream_t s);
+
+// device Prim (MIST)
+struct MistArgs {
+    int nD, V;
+    const double *s1d, *s2d;
+    double* key;
+    int *upar, *inpos, *state;
+    int* edges;
+    double* mi;
+    int* left;
+    int* flags;
+};
+hipError_t launch_mist(const MistArgs& a, hipStream_t s);
+
+}  // namespace parent

@@ -1,0 +1,302 @@
+// 2D histogram + entropy kernels built on LDS atomics (PARENT.cpp:774-1479).
+//
+// Both kernels read the dof-major uint8 bin indices written by the pre-bin
+// pass, so the reference's two FP64 divisions per pair-frame (PARENT.cpp:803)
+// happen once per dof-frame instead.  Counts are exact uint32 (N < 2^32).
+//
+//  k_pairs_plain   one dof pair per workgroup; per-cell FP64 epilogue in the
+//                  reference's form p = n/(N d1 d2 J1 J2), S = -sum J1 J2 p ln p
+//                  d1 d2 + (occ-1)/(2N).  The baseline GPU kernel.
+//  k_pairs_atomic  a TR x TC tile of dof pairs per workgroup: the tile's row
+//                  and column index rows are streamed once and feed TR*TC LDS
+//                  histograms; fused epilogue S = ln N + c_i + c_j
+//                  - (1/N) sum n ln n + (occ-1)/(2N) over an FP64 n ln n table.
+#include "kernels.h"
+
+namespace parent {
+
+namespace {
+
+__device__ __forceinline__ unsigned byte_of(const uint4& w, int e) {
+    const unsigned v = e < 4 ? w.x : (e < 8 ? w.y : (e < 12 ? w.z : w.w));
+    return (v >> (8 * (e & 3))) & 0xFFu;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ int wave_sum(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// plain: one pair per workgroup
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_pairs_plain(PairInputs in, long long p0, long long p1,
+                                                     const int* __restrict__ plist,
+                                                     const long long* __restrict__ dump_off,
+                                                     int* __restrict__ dump_out) {
+    extern __shared__ unsigned hist[];
+    __shared__ double rs[256];
+    __shared__ int ro[256];
+    const long long p = p0 + blockIdx.x;
+    if (p >= p1) return;
+    int gi, gj;
+    if (plist) {
+        gi = plist[2 * p];
+        gj = plist[2 * p + 1];
+    } else {
+        pair_of_index(p, in.D, &gi, &gj);
+    }
+    const int ti = dof_type(in.nD, gi), tj = dof_type(in.nD, gj);
+    const int Bi = in.B.of(ti), Bj = in.B.of(tj);
+    const int cells = Bi * Bj;
+    for (int c = threadIdx.x; c < cells; c += 256) hist[c] = 0;
+    __syncthreads();
+    const uint32_t* xi = reinterpret_cast<const uint32_t*>(in.bins + (size_t)gi * in.Np);
+    const uint32_t* xj = reinterpret_cast<const uint32_t*>(in.bins + (size_t)gj * in.Np);
+    const int N = in.N;
+    const int nwords = (N + 3) / 4;
+    for (int w = threadIdx.x; w < nwords; w += 256) {
+        const uint32_t a4 = xi[w], b4 = xj[w];
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            if (4 * w + e < N)
+                atomicAdd(&hist[((a4 >> (8 * e)) & 0xFFu) * Bj + ((b4 >> (8 * e)) & 0xFFu)], 1u);
+        }
+    }
+    __syncthreads();
+    if (dump_out) {
+        int* out = dump_out + dump_off[p];
+        for (int c = threadIdx.x; c < cells; c += 256) out[c] = (int)hist[c];
+        return;
+    }
+    const double b1 = (in.vmax[gi] - in.vmin[gi]) / Bi;
+    const double b2 = (in.vmax[gj] - in.vmin[gj]) / Bj;
+    const double* J1 = in.jtab + (size_t)gi * 256;
+    const double* J2 = in.jtab + (size_t)gj * 256;
+    const int numFrames = N;
+    double sum = 0.0;
+    int occ = 0;
+    for (int c = threadIdx.x; c < cells; c += 256) {
+        const unsigned n = hist[c];
+        if (n) {
+            const int k = c / Bj, l = c - k * Bj;
+            const double j1 = J1[k], j2 = J2[l];
+            const double probDens = n / (numFrames * b1 * b2 * j1 * j2);
+            if (probDens > 0) {
+                sum = sum + j1 * j2 * probDens * log(probDens) * b1 * b2;
+                occ = occ + 1;
+            }
+        }
+    }
+    rs[threadIdx.x] = sum;
+    ro[threadIdx.x] = occ;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            rs[threadIdx.x] += rs[threadIdx.x + s];
+            ro[threadIdx.x] += ro[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        in.s2d[par_offset(in.nD, gi, gj)] = -rs[0] + (ro[0] - 1.0) / (2.0 * numFrames);
+}
+
+hipError_t launch_pairs_plain(const PairInputs& in, const CountsDump* dump, hipStream_t s) {
+    const int bmax = max(in.B.b, max(in.B.a, in.B.d));
+    const size_t lds = (size_t)bmax * bmax * sizeof(unsigned);
+    const long long total = dump ? dump->n : (long long)in.D * (in.D - 1) / 2;
+    const long long chunk = 1LL << 23;
+    for (long long p0 = 0; p0 < total; p0 += chunk) {
+        const long long p1 = p0 + chunk < total ? p0 + chunk : total;
+        k_pairs_plain<<<(unsigned)(p1 - p0), 256, lds, s>>>(in, p0, p1, dump ? dump->pairs : nullptr,
+                                                            dump ? dump->offsets : nullptr,
+                                                            dump ? dump->out : nullptr);
+    }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// tiled: TR x TC pairs per workgroup
+// ---------------------------------------------------------------------------
+// tiles: [ntiles][2] = (first row dof, first column dof).  Block b of a launch
+// takes tile (b % 8) * chunk + b / 8 of its range, so the workgroups that the
+// dispatcher places on one XCD (round robin over 8 XCDs) walk one contiguous
+// run of tiles and share its row-dof index rows in that XCD's L2.
+template <int TR, int TC, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_pairs_atomic(PairInputs in, int bmax,
+                                                        const int* __restrict__ tiles, int t_begin,
+                                                        int t_end, const int* __restrict__ dump_begin,
+                                                        const int* __restrict__ dump_entries,
+                                                        int* __restrict__ dump_out) {
+    extern __shared__ unsigned hist[];
+    const int chunk = (t_end - t_begin + 7) / 8;
+    const int t = t_begin + (int)(blockIdx.x % 8) * chunk + (int)(blockIdx.x / 8);
+    if (t >= t_end || (int)(blockIdx.x / 8) >= chunk) return;
+    const int r0 = tiles[2 * t], c0 = tiles[2 * t + 1];
+    const int D = in.D, N = in.N, nD = in.nD;
+    const int slot = bmax * bmax;
+
+    int gr[TR], gc[TC], Bc[TC];
+#pragma unroll
+    for (int r = 0; r < TR; r++) gr[r] = r0 + r;
+#pragma unroll
+    for (int c = 0; c < TC; c++) {
+        gc[c] = c0 + c;
+        Bc[c] = gc[c] < D ? in.B.of(dof_type(nD, gc[c])) : 1;
+    }
+    unsigned valid = 0;  // bit r*TC+c
+#pragma unroll
+    for (int r = 0; r < TR; r++)
+#pragma unroll
+        for (int c = 0; c < TC; c++)
+            if (gr[r] < D && gc[c] < D && gr[r] < gc[c]) valid |= 1u << (r * TC + c);
+
+    for (int i = threadIdx.x; i < TR * TC * slot; i += BLOCK) hist[i] = 0;
+    __syncthreads();
+
+    const int per_iter = BLOCK * 16;
+    const int niter = (N + per_iter - 1) / per_iter;
+    const int nfull = N / per_iter;
+    for (int it = 0; it < niter; it++) {
+        const int f = it * per_iter + threadIdx.x * 16;
+        uint4 R[TR], C[TC];
+#pragma unroll
+        for (int r = 0; r < TR; r++)
+            R[r] = (gr[r] < D && f < in.Np)
+                       ? *reinterpret_cast<const uint4*>(in.bins + (size_t)gr[r] * in.Np + f)
+                       : make_uint4(0, 0, 0, 0);
+#pragma unroll
+        for (int c = 0; c < TC; c++)
+            C[c] = (gc[c] < D && f < in.Np)
+                       ? *reinterpret_cast<const uint4*>(in.bins + (size_t)gc[c] * in.Np + f)
+                       : make_uint4(0, 0, 0, 0);
+        const int nvalid = it < nfull ? 16 : min(16, max(0, N - f));
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+            if (e < nvalid) {
+#pragma unroll
+                for (int r = 0; r < TR; r++) {
+                    const unsigned a = byte_of(R[r], e);
+#pragma unroll
+                    for (int c = 0; c < TC; c++) {
+                        if (valid & (1u << (r * TC + c)))
+                            atomicAdd(&hist[(r * TC + c) * slot + a * Bc[c] + byte_of(C[c], e)], 1u);
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+
+    if (dump_out) {
+        // entries of this tile: (slot, offset) with offset split in two ints
+        const int e0 = dump_begin[t], e1 = dump_begin[t + 1];
+        for (int e = e0; e < e1; e++) {
+            const int sl = dump_entries[3 * e];
+            const long long off = ((long long)dump_entries[3 * e + 1] << 31) | (long long)dump_entries[3 * e + 2];
+            const int r = sl / TC, c = sl % TC;
+            const int cells = in.B.of(dof_type(nD, gr[r])) * Bc[c];
+            for (int i = threadIdx.x; i < cells; i += BLOCK) dump_out[off + i] = (int)hist[sl * slot + i];
+        }
+        return;
+    }
+
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    constexpr int kWaves = BLOCK / 64;
+    for (int sl = wid; sl < TR * TC; sl += kWaves) {
+        if (!(valid & (1u << sl))) continue;
+        const int r = sl / TC, c = sl % TC;
+        const int gi = gr[r], gj = gc[c];
+        const int cells = in.B.of(dof_type(nD, gi)) * Bc[c];
+        const unsigned* h = hist + sl * slot;
+        double T = 0.0;
+        int occ = 0;
+        for (int i = lane; i < cells; i += 64) {
+            const unsigned n = h[i];
+            if (n) {
+                T += in.nlogn[n];
+                occ++;
+            }
+        }
+        T = wave_sum(T);
+        occ = wave_sum(occ);
+        if (lane == 0)
+            in.s2d[par_offset(nD, gi, gj)] =
+                ((in.logN + in.c2[gi]) + in.c2[gj] + (occ - 1.0) / (2.0 * N)) - T / N;
+    }
+}
+
+namespace {
+
+struct AtomicVariant {
+    AtomicConfig cfg;
+    const void* fn;
+};
+
+#define PARENT_ATOMIC_VARIANT(TR, TC, BLK) \
+    AtomicVariant{AtomicConfig{TR, TC, BLK}, reinterpret_cast<const void*>(&k_pairs_atomic<TR, TC, BLK>)}
+
+const AtomicVariant kAtomicVariants[] = {
+    PARENT_ATOMIC_VARIANT(1, 1, 256),  PARENT_ATOMIC_VARIANT(2, 2, 256),
+    PARENT_ATOMIC_VARIANT(2, 2, 512),  PARENT_ATOMIC_VARIANT(3, 3, 512),
+    PARENT_ATOMIC_VARIANT(3, 3, 1024), PARENT_ATOMIC_VARIANT(4, 4, 1024),
+    PARENT_ATOMIC_VARIANT(1, 4, 256),  PARENT_ATOMIC_VARIANT(2, 4, 512),
+    PARENT_ATOMIC_VARIANT(1, 8, 512),  PARENT_ATOMIC_VARIANT(4, 4, 512),
+};
+
+template <int TR, int TC, int BLK>
+void launch_one(dim3 grid, size_t lds, hipStream_t s, const PairInputs& in, int bmax, const int* tiles,
+                int t0, int t1, const int* db, const int* de, int* dout) {
+    k_pairs_atomic<TR, TC, BLK><<<grid, BLK, lds, s>>>(in, bmax, tiles, t0, t1, db, de, dout);
+}
+
+}  // namespace
+
+int atomic_config_count() { return (int)(sizeof(kAtomicVariants) / sizeof(kAtomicVariants[0])); }
+AtomicConfig atomic_config(int id) { return kAtomicVariants[id].cfg; }
+size_t atomic_lds_bytes(const AtomicConfig& c, int bmax) {
+    return (size_t)c.tr * c.tc * bmax * bmax * sizeof(unsigned);
+}
+
+hipError_t launch_pairs_atomic(const PairInputs& in, int id, int bmax, const int* tiles, int ntiles,
+                               const int* dump_begin, const int* dump_entries, int* dump_out,
+                               hipStream_t s) {
+    const AtomicConfig c = kAtomicVariants[id].cfg;
+    const size_t lds = atomic_lds_bytes(c, bmax);
+    hipError_t e = hipFuncSetAttribute(kAtomicVariants[id].fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    const int per_launch = 1 << 22;
+    for (int t0 = 0; t0 < ntiles; t0 += per_launch) {
+        const int t1 = t0 + per_launch < ntiles ? t0 + per_launch : ntiles;
+        const int chunk = (t1 - t0 + 7) / 8;
+        dim3 grid((unsigned)chunk * 8);
+        switch (id) {
+#define CASE(I, TR, TC, BLK) \
+    case I: launch_one<TR, TC, BLK>(grid, lds, s, in, bmax, tiles, t0, t1, dump_begin, dump_entries, dump_out); break;
+            CASE(0, 1, 1, 256)
+            CASE(1, 2, 2, 256)
+            CASE(2, 2, 2, 512)
+            CASE(3, 3, 3, 512)
+            CASE(4, 3, 3, 1024)
+            CASE(5, 4, 4, 1024)
+            CASE(6, 1, 4, 256)
+            CASE(7, 2, 4, 512)
+            CASE(8, 1, 8, 512)
+            CASE(9, 4, 4, 512)
+#undef CASE
+            default: return hipErrorInvalidValue;
+        }
+    }
+    return hipGetLastError();
+}
+
+}  // namespace parent

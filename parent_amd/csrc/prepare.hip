@@ -1,0 +1,446 @@
+// Pre-binning kernels: per-dof extrema, dihedral modfit, bin indices and the
+// per-dof 1D statistics (PARENT.cpp:503-772).
+//
+// Compiled with -ffp-contract=off: the range rules, the modfit shift and the
+// bin-index divisions must round exactly as the reference's FP64 code does, so
+// that every index is bit-identical (no FMA contraction, IEEE division).
+//
+// Data layout in HBM
+//   values   frame-major [N][D] float or double (the trajectory as handed in)
+//   bins2    dof-major  [D][Np] uint8, Np = N rounded up to 4096; frames >= N
+//            hold 0xFF.  One row per dof so a pair kernel streams two rows.
+//   bins1    same layout for the 1D bin counts, only when they differ from the
+//            2D ones for some dof type.
+#include "kernels.h"
+
+namespace parent {
+
+__device__ __forceinline__ unsigned long long ord_enc(double x) {
+    unsigned long long u = (unsigned long long)__double_as_longlong(x);
+    return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double ord_dec(unsigned long long e) {
+    unsigned long long u = (e >> 63) ? (e & 0x7FFFFFFFFFFFFFFFull) : ~e;
+    return __longlong_as_double((long long)u);
+}
+
+// x + modFit - 2*pi*int((x+modFit)/(2*pi))   (PARENT.cpp:610-612)
+__device__ __forceinline__ double modfit_shift(double x, double mf, double twopi) {
+    const double s = x + mf;
+    const int k = (int)(s / twopi);
+    return s - twopi * (double)k;
+}
+
+__global__ void k_fill_u64(unsigned long long* p, size_t n, unsigned long long v) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_fill_u32(unsigned* p, size_t n, unsigned v) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// Pass 1: min/max of every dof over a slice of frames (exact in the input
+// precision), plus the 100-bin modfit histogram of each dihedral
+// (PARENT.cpp:513-521, 537-545, 568-572).  Block = 64 dofs x 4 frame lanes;
+// one wave reads 64 consecutive values of a frame row.
+template <class T>
+__global__ __launch_bounds__(256) void k_scan_raw(const T* __restrict__ v, int N, int D, int nD,
+                                                  int fpb, double mf_bin,
+                                                  unsigned long long* __restrict__ emin,
+                                                  unsigned long long* __restrict__ emax,
+                                                  unsigned* __restrict__ mfh, int* flags) {
+    __shared__ unsigned hist[64 * kModfitBins];
+    __shared__ double smin[256], smax[256];
+    const int dl = threadIdx.x & 63, fl = threadIdx.x >> 6;
+    const int g = blockIdx.x * 64 + dl;
+    const int dih0 = 2 * nD + 3;
+    const bool blk_dih = (int)(blockIdx.x * 64 + 63) >= dih0;
+    if (blk_dih)
+        for (int i = threadIdx.x; i < 64 * kModfitBins; i += 256) hist[i] = 0;
+    __syncthreads();
+    double mn = 1.0e300, mx = -1.0e300;
+    const int f0 = blockIdx.y * fpb, f1 = min(N, f0 + fpb);
+    if (g < D) {
+        const bool isd = g >= dih0;
+        bool bad = false;
+        for (int f = f0 + fl; f < f1; f += 4) {
+            const double x = (double)v[(size_t)f * D + g];
+            mn = x < mn ? x : mn;
+            mx = x > mx ? x : mx;
+            if (isd) {
+                const int k = (int)(x / mf_bin);
+                if (k < 0 || k >= kModfitBins) bad = true;
+                else atomicAdd(&hist[dl * kModfitBins + k], 1u);
+            }
+        }
+        if (bad) atomicOr(flags, kFlagRange);
+    }
+    smin[threadIdx.x] = mn;
+    smax[threadIdx.x] = mx;
+    __syncthreads();
+    if (fl == 0 && g < D) {
+        for (int q = 1; q < 4; q++) {
+            const double a = smin[dl + 64 * q], b = smax[dl + 64 * q];
+            mn = a < mn ? a : mn;
+            mx = b > mx ? b : mx;
+        }
+        atomicMin(&emin[g], ord_enc(mn));
+        atomicMax(&emax[g], ord_enc(mx));
+    }
+    if (blk_dih) {
+        for (int i = threadIdx.x; i < 64 * kModfitBins; i += 256) {
+            const int gg = blockIdx.x * 64 + i / kModfitBins;
+            if (gg >= dih0 && gg < D && hist[i]) atomicAdd(&mfh[(size_t)(gg - dih0) * kModfitBins + i % kModfitBins], hist[i]);
+        }
+    }
+}
+
+// Pass 2: range rules of bonds and angles, and the modfit shift of each
+// dihedral from its 100-bin histogram (PARENT.cpp:522-533, 546-562, 573-609).
+__global__ void k_finalize_ranges(int D, int nD, double pi, double mf_bin,
+                                  const unsigned long long* __restrict__ emin,
+                                  const unsigned long long* __restrict__ emax,
+                                  const unsigned* __restrict__ mfh, double* vmin, double* vmax,
+                                  double* modfit, int* flags) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= D) return;
+    const int t = dof_type(nD, g);
+    if (t == 2) {
+        const unsigned* histo = mfh + (size_t)(g - (2 * nD + 3)) * kModfitBins;
+        bool zeroExists = false;
+        for (int k = 0; k < kModfitBins; k++) zeroExists = zeroExists || (histo[k] == 0);
+        long long longest;
+        int longestPos, current, currentPos = 0;
+        if (zeroExists) {
+            longest = 0;
+            current = 0;
+            longestPos = -1;
+            for (int k = 0; k < 2 * kModfitBins; k++) {
+                const int l = k % kModfitBins;
+                if ((current == 0) && (histo[l] == 0)) {
+                    current = 1;
+                    currentPos = k;
+                }
+                if ((current > 0) && (histo[l] == 0)) current += 1;
+                if ((current > 0) && (histo[l] != 0)) {
+                    if (current > longest) {
+                        longest = current;
+                        longestPos = currentPos;
+                    }
+                    current = 0;
+                }
+            }
+        } else {
+            longestPos = 0;
+            longest = histo[0];
+            for (int k = 0; k < kModfitBins; k++) {
+                if ((long long)histo[k] < longest) {
+                    longest = histo[k];
+                    longestPos = k;
+                }
+            }
+        }
+        modfit[g] = 2 * pi - (longestPos + 0.5) * mf_bin;
+        return;
+    }
+    double tmpMin = ord_dec(emin[g]), tmpMax = ord_dec(emax[g]);
+    if (t == 0) {
+        if ((tmpMin < 0.0) || (tmpMax < 0.0)) atomicOr(flags, kFlagNegBond);
+        tmpMin -= 0.000000005;
+        tmpMax += 0.000000005;
+        if (tmpMin < 0.0) tmpMin = 0.0;
+        if ((tmpMax - tmpMin) < 1.0e-4) tmpMax += 0.05;
+    } else {
+        tmpMin -= 0.000000005;
+        tmpMax += 0.000000005;
+        if (tmpMin < 0) tmpMin = 0;
+        if (tmpMax > pi) tmpMax = pi;
+        if ((tmpMax - tmpMin) < 1.0e-4) tmpMax += 0.05;
+    }
+    vmin[g] = tmpMin;
+    vmax[g] = tmpMax;
+    modfit[g] = 0.0;
+}
+
+// Pass 3: extrema of the shifted dihedrals (PARENT.cpp:614-623), FP64.
+template <class T>
+__global__ __launch_bounds__(256) void k_scan_dihedral(const T* __restrict__ v, int N, int D, int nD,
+                                                       int fpb, double twopi,
+                                                       const double* __restrict__ modfit,
+                                                       unsigned long long* __restrict__ emin,
+                                                       unsigned long long* __restrict__ emax) {
+    __shared__ double smin[256], smax[256];
+    const int dl = threadIdx.x & 63, fl = threadIdx.x >> 6;
+    const int dih0 = 2 * nD + 3;
+    const int g = dih0 + blockIdx.x * 64 + dl;
+    double mn = 1.0e300, mx = -1.0e300;
+    const int f0 = blockIdx.y * fpb, f1 = min(N, f0 + fpb);
+    if (g < D) {
+        const double mf = modfit[g];
+        for (int f = f0 + fl; f < f1; f += 4) {
+            const double x = modfit_shift((double)v[(size_t)f * D + g], mf, twopi);
+            mn = x < mn ? x : mn;
+            mx = x > mx ? x : mx;
+        }
+    }
+    smin[threadIdx.x] = mn;
+    smax[threadIdx.x] = mx;
+    __syncthreads();
+    if (fl == 0 && g < D) {
+        for (int q = 1; q < 4; q++) {
+            const double a = smin[dl + 64 * q], b = smax[dl + 64 * q];
+            mn = a < mn ? a : mn;
+            mx = b > mx ? b : mx;
+        }
+        atomicMin(&emin[g], ord_enc(mn));
+        atomicMax(&emax[g], ord_enc(mx));
+    }
+}
+
+// Pass 4: dihedral range rules (PARENT.cpp:624-636).
+__global__ void k_finalize_dihedral(int D, int nD, double twopi,
+                                    const unsigned long long* __restrict__ emin,
+                                    const unsigned long long* __restrict__ emax, double* vmin,
+                                    double* vmax) {
+    const int g = 2 * nD + 3 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= D) return;
+    double tmpMin = ord_dec(emin[g]), tmpMax = ord_dec(emax[g]);
+    tmpMin -= 0.000000005;
+    tmpMax += 0.000000005;
+    if (tmpMin < 0.0) tmpMin = 0.0;
+    if (tmpMax > twopi) tmpMax = twopi;
+    if ((tmpMax - tmpMin) < 1.0e-4) tmpMax += 0.05;
+    vmin[g] = tmpMin;
+    vmax[g] = tmpMax;
+}
+
+// Pass 5: bin indices int((x-min)/((max-min)/B)) for every (frame, dof)
+// (PARENT.cpp:654-656 and the 2D fills, 802-804 ...), written transposed to
+// the dof-major rows through an LDS tile of 64 dofs x 256 frames so both the
+// frame-major reads and the dof-major writes are coalesced.
+template <class T>
+__global__ __launch_bounds__(256) void k_prebin(const T* __restrict__ v, int N, int Np, int D, int nD,
+                                                const double* __restrict__ vmin,
+                                                const double* __restrict__ vmax,
+                                                const double* __restrict__ modfit, double twopi,
+                                                Bins3 B2, Bins3 B1, int need1,
+                                                uint8_t* __restrict__ bins2, uint8_t* __restrict__ bins1,
+                                                int* flags) {
+    constexpr int kRow = 260;  // 256 frames + 4: rows start in different LDS banks
+    __shared__ uint8_t tile2[64 * kRow];
+    __shared__ uint8_t tile1[64 * kRow];
+    const int dl = threadIdx.x & 63, fl = threadIdx.x >> 6;
+    const int d0 = blockIdx.x * 64, f0 = blockIdx.y * 256;
+    const int g = d0 + dl;
+    const int dih0 = 2 * nD + 3;
+    double mn = 0, bs2 = 1, bs1 = 1, mf = 0;
+    int nb2 = 1, nb1 = 1;
+    bool isd = false, sep1 = false;
+    if (g < D) {
+        const int t = dof_type(nD, g);
+        mn = vmin[g];
+        nb2 = B2.of(t);
+        nb1 = B1.of(t);
+        bs2 = (vmax[g] - vmin[g]) / nb2;
+        bs1 = (vmax[g] - vmin[g]) / nb1;
+        isd = g >= dih0;
+        mf = isd ? modfit[g] : 0.0;
+        sep1 = need1 && nb1 != nb2;
+    }
+    bool bad = false;
+    for (int r = fl; r < 256; r += 4) {
+        const int f = f0 + r;
+        uint8_t i2 = kPadBin, i1 = kPadBin;
+        if (g < D && f < N) {
+            double x = (double)v[(size_t)f * D + g];
+            if (isd) x = modfit_shift(x, mf, twopi);
+            int k = (int)((x - mn) / bs2);
+            if (k < 0 || k >= nb2) {
+                bad = true;
+                k = k < 0 ? 0 : nb2 - 1;
+            }
+            i2 = (uint8_t)k;
+            if (sep1) {
+                int k1 = (int)((x - mn) / bs1);
+                if (k1 < 0 || k1 >= nb1) {
+                    bad = true;
+                    k1 = k1 < 0 ? 0 : nb1 - 1;
+                }
+                i1 = (uint8_t)k1;
+            } else {
+                i1 = i2;
+            }
+        }
+        tile2[dl * kRow + r] = i2;
+        tile1[dl * kRow + r] = i1;
+    }
+    if (bad) atomicOr(flags, kFlagRange);
+    __syncthreads();
+    for (int w = threadIdx.x; w < 64 * 64; w += 256) {
+        const int dof = w >> 6, wr = w & 63;
+        const int gg = d0 + dof;
+        if (gg >= D) continue;
+        const uint32_t word2 = *reinterpret_cast<const uint32_t*>(&tile2[dof * kRow + 4 * wr]);
+        *reinterpret_cast<uint32_t*>(&bins2[(size_t)gg * Np + f0 + 4 * wr]) = word2;
+        if (need1) {
+            const uint32_t word1 = *reinterpret_cast<const uint32_t*>(&tile1[dof * kRow + 4 * wr]);
+            *reinterpret_cast<uint32_t*>(&bins1[(size_t)gg * Np + f0 + 4 * wr]) = word1;
+        }
+    }
+}
+
+// Pass 6: per-dof histograms, the 1D entropy with the reference's formula
+// (PARENT.cpp:650-671 bonds, 694-715 angles, 738-757 dihedrals), and the
+// per-dof constants of the 2D epilogue:
+//   jtab[g][k] = Jacobian of 2D bin k (blen^2, sin(theta) or 1, with the
+//                reference's bin-centre recurrences, PARENT.cpp:808-819)
+//   c2[g]      = ln(binsize2) + (1/N) sum_k h2[k] ln J_k
+// With these, S_ij = ln N + c2[i] + c2[j] - (1/N) sum n ln n + (occ-1)/(2N),
+// which is the reference epilogue regrouped (the marginal of a 2D histogram
+// is the dof's 1D histogram at the 2D bin count).
+__global__ __launch_bounds__(256) void k_dof_stats(const uint8_t* __restrict__ bins2,
+                                                   const uint8_t* __restrict__ bins1, int N, int Np,
+                                                   int nD, Bins3 B1, Bins3 B2,
+                                                   const double* __restrict__ vmin,
+                                                   const double* __restrict__ vmax, double* s1d,
+                                                   double* c2, double* jtab) {
+    __shared__ unsigned h2[256], h1[256];
+    const int g = blockIdx.x;
+    const int t = dof_type(nD, g);
+    const bool sep1 = bins1 != nullptr && B1.of(t) != B2.of(t);
+    h2[threadIdx.x] = 0;
+    h1[threadIdx.x] = 0;
+    __syncthreads();
+    const uint4* r2 = reinterpret_cast<const uint4*>(bins2 + (size_t)g * Np);
+    const uint4* r1 = reinterpret_cast<const uint4*>((sep1 ? bins1 : bins2) + (size_t)g * Np);
+    for (int i = threadIdx.x; i < Np / 16; i += 256) {
+        const uint4 w = r2[i];
+        const unsigned ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const unsigned b = (ws[q >> 2] >> (8 * (q & 3))) & 0xFFu;
+            if (b != kPadBin) atomicAdd(&h2[b], 1u);
+        }
+        if (sep1) {
+            const uint4 u = r1[i];
+            const unsigned us[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                const unsigned b = (us[q >> 2] >> (8 * (q & 3))) & 0xFFu;
+                if (b != kPadBin) atomicAdd(&h1[b], 1u);
+            }
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const unsigned* histo = sep1 ? h1 : h2;
+    const int numFrames = N;
+    {
+        const int bins = B1.of(t);
+        const double binsize = (vmax[g] - vmin[g]) / (bins);
+        int occupbins = 0;
+        double plnpsum = 0, probDens;
+        if (t == 0) {
+            double blen = vmin[g] + (binsize / 2.0);
+            for (int k = 0; k < bins; k++) {
+                probDens = histo[k] / (numFrames * binsize * blen * blen);
+                if (probDens > 0) {
+                    plnpsum = plnpsum + blen * blen * probDens * log(probDens);
+                    occupbins = occupbins + 1;
+                }
+                blen += binsize;
+            }
+        } else if (t == 1) {
+            double theta = vmin[g] + (binsize / 2.0);
+            for (int k = 0; k < bins; k++) {
+                probDens = histo[k] / (numFrames * binsize * sin(theta));
+                if (probDens > 0) {
+                    plnpsum = plnpsum + sin(theta) * probDens * log(probDens);
+                    occupbins = occupbins + 1;
+                }
+                theta += binsize;
+            }
+        } else {
+            for (int k = 0; k < bins; k++) {
+                probDens = histo[k] / (numFrames * binsize);
+                if (probDens > 0) {
+                    plnpsum = plnpsum + probDens * log(probDens);
+                    occupbins = occupbins + 1;
+                }
+            }
+        }
+        plnpsum = -plnpsum * binsize;
+        s1d[g] = plnpsum + (occupbins - 1.0) / (2.0 * numFrames);
+    }
+    {
+        const int bins = B2.of(t);
+        const double binsize = (vmax[g] - vmin[g]) / bins;
+        double acc = 0.0;
+        double* jt = jtab + (size_t)g * 256;
+        if (t == 0) {
+            double blen = vmin[g] + binsize / 2.0;
+            for (int k = 0; k < bins; k++) {
+                const double J = blen * blen;
+                jt[k] = J;
+                if (h2[k]) acc += (double)h2[k] * log(J);
+                blen = blen + binsize;
+            }
+        } else if (t == 1) {
+            double theta = vmin[g] + binsize / 2.0;
+            for (int k = 0; k < bins; k++) {
+                const double J = sin(theta);
+                jt[k] = J;
+                if (h2[k]) acc += (double)h2[k] * log(J);
+                theta = theta + binsize;
+            }
+        } else {
+            for (int k = 0; k < bins; k++) jt[k] = 1.0;
+        }
+        c2[g] = log(binsize) + acc / numFrames;
+    }
+}
+
+// n*ln(n) for n = 0..N in FP64: the 2D epilogue sums table entries.
+__global__ void k_nlogn(int N, double* tab) {
+    for (int n = blockIdx.x * blockDim.x + threadIdx.x; n <= N; n += gridDim.x * blockDim.x)
+        tab[n] = n == 0 ? 0.0 : (double)n * log((double)n);
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+
+hipError_t launch_prepare(const PrepareArgs& a, hipStream_t s) {
+    const int D = a.D, N = a.N, nD = a.nD;
+    const int nDih = nD;
+    const double pi = a.pi, twopi = 2 * pi;
+    const double mf_bin = (2 * pi + 0.000000005) / kModfitBins;
+    k_fill_u64<<<64, 256, 0, s>>>(a.emin, (size_t)D, ~0ull);
+    k_fill_u64<<<64, 256, 0, s>>>(a.emax, (size_t)D, 0ull);
+    if (nDih > 0) k_fill_u32<<<64, 256, 0, s>>>(a.mfhist, (size_t)nDih * kModfitBins, 0u);
+    k_fill_u32<<<1, 32, 0, s>>>((unsigned*)a.flags, 1, 0u);
+    const int fpb = 2048;
+    dim3 g1((D + 63) / 64, (N + fpb - 1) / fpb);
+    if (a.double_prec)
+        k_scan_raw<double><<<g1, 256, 0, s>>>((const double*)a.values, N, D, nD, fpb, mf_bin, a.emin, a.emax, a.mfhist, a.flags);
+    else
+        k_scan_raw<float><<<g1, 256, 0, s>>>((const float*)a.values, N, D, nD, fpb, mf_bin, a.emin, a.emax, a.mfhist, a.flags);
+    k_finalize_ranges<<<(D + 255) / 256, 256, 0, s>>>(D, nD, pi, mf_bin, a.emin, a.emax, a.mfhist, a.vmin, a.vmax, a.modfit, a.flags);
+    if (nDih > 0) {
+        dim3 g3((nDih + 63) / 64, (N + fpb - 1) / fpb);
+        if (a.double_prec)
+            k_scan_dihedral<double><<<g3, 256, 0, s>>>((const double*)a.values, N, D, nD, fpb, twopi, a.modfit, a.emin, a.emax);
+        else
+            k_scan_dihedral<float><<<g3, 256, 0, s>>>((const float*)a.values, N, D, nD, fpb, twopi, a.modfit, a.emin, a.emax);
+        k_finalize_dihedral<<<(nDih + 255) / 256, 256, 0, s>>>(D, nD, twopi, a.emin, a.emax, a.vmin, a.vmax);
+    }
+    dim3 g5((D + 63) / 64, a.Np / 256);
+    if (a.double_prec)
+        k_prebin<double><<<g5, 256, 0, s>>>((const double*)a.values, N, a.Np, D, nD, a.vmin, a.vmax, a.modfit, twopi, a.B2, a.B1, a.bins1 != nullptr, a.bins2, a.bins1, a.flags);
+    else
+        k_prebin<float><<<g5, 256, 0, s>>>((const float*)a.values, N, a.Np, D, nD, a.vmin, a.vmax, a.modfit, twopi, a.B2, a.B1, a.bins1 != nullptr, a.bins2, a.bins1, a.flags);
+    k_dof_stats<<<D, 256, 0, s>>>(a.bins2, a.bins1, N, a.Np, nD, a.B1, a.B2, a.vmin, a.vmax, a.s1d, a.c2, a.jtab);
+    k_nlogn<<<(N + 256) / 256, 256, 0, s>>>(N, a.nlogn);
+    return hipGetLastError();
+}
+
+}  // namespace parent

@@ -1,0 +1,218 @@
+"""HIP engine vs CPU oracle on the same seeded trajectories (needs an MI355X).
+
+Bars:
+  * ranges (min, max, modfit shift) and every bin index: bit-exact;
+  * histogram counts of every checked pair, each fill path: bit-exact;
+  * S1D: same formula as the reference; only libm's log/sin may differ by an ulp,
+    so |dS1D| <= 1e-12;
+  * S2D: |dS2D| <= 1e-10 and per-pair MI within 1e-6 relative of the FP64
+    oracle with an absolute floor of 1e-11 (MI ~ 0 for independent pairs);
+  * MIST from identical S1D/S2D: identical edge list, MI bits and totals.
+"""
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+import pytest
+
+import oracle_api as O
+
+pytestmark = pytest.mark.gpu
+
+MI_REL, MI_ABS, S_ABS = 1e-6, 1e-11, 1e-10
+
+
+def _pa():
+    import parent_amd
+    return parent_amd
+
+
+def _case(atoms, frames, seed):
+    pa = _pa()
+    vals = pa.synth_values(atoms, frames, seed)
+    return vals, atoms - 3
+
+
+def _oracle_full(vals, nD, bins):
+    N = vals.shape[0]
+    return O.compute(nD, N, vals.T.astype(np.float64), bins)
+
+
+def _mi(nD, s1, s2):
+    D = 3 * nD + 3
+    gi, gj = np.triu_indices(D, 1)
+    pos = np.array([O.lib().oracle_par_pos(nD, int(a), int(b)) for a, b in zip(gi, gj)])
+    return s1[gi] + s1[gj] - s2[pos]
+
+
+@pytest.fixture(scope="module")
+def engine():
+    pa = _pa()
+    print("engine: create on device 0", flush=True)
+    e = pa.Engine(0)
+    yield e
+    e.close()
+
+
+CASES = [
+    (12, 1000, 3, [50] * 6),
+    (30, 5000, 11, [50] * 6),
+    (20, 3000, 5, [7, 13, 29, 5, 40, 17]),
+    (25, 4097, 8, [10] * 6),
+    (16, 2000, 9, [100, 100, 100, 100, 100, 100]),
+]
+
+
+@pytest.mark.parametrize("atoms,frames,seed,bins", CASES)
+def test_prepare_bit_exact(engine, atoms, frames, seed, bins):
+    print(f"prepare atoms={atoms} frames={frames} seed={seed} bins={bins}", flush=True)
+    vals, nD = _case(atoms, frames, seed)
+    engine.set_trajectory(vals)
+    engine.prepare(bins)
+    s1o, _, mno, mxo, mfo, shifted = _oracle_full(vals, nD, bins)
+    mn, mx, mf = engine.ranges()
+    assert np.array_equal(mn.view(np.uint64), mno.view(np.uint64))
+    assert np.array_equal(mx.view(np.uint64), mxo.view(np.uint64))
+    assert np.array_equal(mf.view(np.uint64), mfo.view(np.uint64))
+    gb = engine.bins()
+    D = 3 * nD + 3
+    for g in range(D):
+        t = O.type_of(nD, g)
+        idx, bad = O.bin_index(shifted[g], mno[g], mxo[g], bins[3 + t])
+        assert bad == 0
+        assert np.array_equal(gb[g].astype(np.int32), idx), f"bin indices differ for dof {g}"
+    s1 = engine.s1d()
+    assert np.max(np.abs(s1 - s1o)) <= 1e-12
+
+
+@pytest.mark.parametrize("method", [0, 1])
+@pytest.mark.parametrize("atoms,frames,seed,bins", CASES[:4])
+def test_counts_bit_exact(engine, method, atoms, frames, seed, bins):
+    print(f"counts method={method} atoms={atoms} frames={frames}", flush=True)
+    vals, nD = _case(atoms, frames, seed)
+    engine.set_trajectory(vals)
+    engine.prepare(bins)
+    D = 3 * nD + 3
+    rng = np.random.default_rng(seed)
+    pairs = set()
+    # every type combination, the first/last dofs and random pairs
+    pairs.update([(0, 1), (0, D - 1), (D - 2, D - 1), (1, nD + 2), (nD + 2, 2 * nD + 3), (2 * nD + 3, 2 * nD + 4)])
+    while len(pairs) < 40:
+        a, b = sorted(rng.choice(D, 2, replace=False))
+        pairs.add((int(a), int(b)))
+    pairs = sorted(pairs)
+    got = engine.pair_counts(pairs, method, bins[3:])
+    _, _, mno, mxo, _, shifted = _oracle_full(vals, nD, bins)
+    for (gi, gj), h in zip(pairs, got):
+        ti, tj = O.type_of(nD, gi), O.type_of(nD, gj)
+        _, ho = O.entropy2d(ti, shifted[gi], mno[gi], mxo[gi], bins[3 + ti],
+                            tj, shifted[gj], mno[gj], mxo[gj], bins[3 + tj], counts=True)
+        assert np.array_equal(h.astype(np.int64), ho), f"counts differ for pair {(gi, gj)}"
+        assert h.sum() == frames
+
+
+@pytest.mark.parametrize("method", [0, 1])
+@pytest.mark.parametrize("atoms,frames,seed,bins", CASES)
+def test_entropies_vs_oracle(engine, method, atoms, frames, seed, bins):
+    print(f"entropies method={method} atoms={atoms} frames={frames} bins={bins}", flush=True)
+    vals, nD = _case(atoms, frames, seed)
+    engine.set_trajectory(vals)
+    engine.prepare(bins)
+    engine.pairs(method)
+    s1, s2 = engine.s1d(), engine.s2d()
+    s1o, s2o, *_ = _oracle_full(vals, nD, bins)
+    assert np.max(np.abs(s2 - s2o)) <= S_ABS
+    mi, mio = _mi(nD, s1, s2), _mi(nD, s1o, s2o)
+    err = np.abs(mi - mio)
+    assert np.all(err <= MI_REL * np.abs(mio) + MI_ABS), f"max MI err {err.max():.3e}"
+    st = engine.stats()
+    print(f"  method used {st['method']} variant {st['kernel_variant']} pairs {st['ms_pairs']:.3f} ms "
+          f"max|dS2D| {np.max(np.abs(s2 - s2o)):.2e} max|dMI| {err.max():.2e}", flush=True)
+
+
+@pytest.mark.parametrize("atoms,frames,seed", [(12, 1000, 3), (40, 3000, 21)])
+def test_mist_identical(engine, atoms, frames, seed):
+    print(f"mist atoms={atoms} frames={frames}", flush=True)
+    vals, nD = _case(atoms, frames, seed)
+    bins = [50] * 6
+    s1o, s2o, *_ = _oracle_full(vals, nD, bins)
+    engine.set_entropies(nD, s1o, s2o)
+    edges, mi, left, tot = engine.mist()
+    eo, mio, lo, to = O.mist(nD, s1o, s2o)
+    assert np.array_equal(edges, eo)
+    assert np.array_equal(mi.view(np.uint64), mio.view(np.uint64))
+    assert np.array_equal(left, lo)
+    assert np.array_equal(tot.view(np.uint64), to.view(np.uint64))
+
+
+def test_mist_ties_follow_reference_scan(engine):
+    """Many exactly equal MI values: the tie rule must reproduce the reference's scan order."""
+    nD = 6
+    D = 3 * nD + 3
+    s1 = np.zeros(D)
+    rng = np.random.default_rng(1)
+    P = D * (D - 1) // 2
+    s2 = -rng.integers(0, 3, P).astype(np.float64)  # MI in {0, 1, 2}: heavy ties
+    engine.set_entropies(nD, s1, s2)
+    edges, mi, left, tot = engine.mist()
+    eo, mio, lo, to = O.mist(nD, s1, s2)
+    assert np.array_equal(edges, eo)
+    assert np.array_equal(mi, mio)
+    assert np.array_equal(tot, to)
+
+
+def test_drop_in_executables(tmp_path):
+    """PARENT_hip.x / get_PAR_MIST_hip.x against the oracle's PARENT.x restatement."""
+    pa = _pa()
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    bat = str(tmp_path / "chain.bat")
+    pa.synth_write_bat(bat, 24, 2500, 17)
+    ours, theirs = str(tmp_path / "gpu.par"), str(tmp_path / "oracle.par")
+    bins = ["50", "50", "50", "50", "50", "50"]
+    print("run PARENT_hip.x", flush=True)
+    r = subprocess.run([os.path.join(root, "parent_amd", "bin", "PARENT_hip.x"), bat, ours] + bins,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.rstrip().endswith("PROGRAM FINISHED SUCCESSFULLY.")
+    r2 = subprocess.run([O.ORACLE_EXE, bat, theirs] + bins, capture_output=True, text=True, timeout=120)
+    assert r2.returncode == 0, r2.stderr
+    a, b = open(ours, "rb").read(), open(theirs, "rb").read()
+    assert len(a) == len(b)
+    nD, N, binsr, s1, s2 = pa.read_par(ours)
+    _, _, _, s1o, s2o = pa.read_par(theirs)
+    hdr = len(a) - 8 * (s1.size + s2.size)
+    assert a[:hdr] == b[:hdr], ".par headers differ"
+    assert np.max(np.abs(s1 - s1o)) <= 1e-12 and np.max(np.abs(s2 - s2o)) <= S_ABS
+    # MIST on the oracle's .par: same text as the oracle's MIST
+    txt = str(tmp_path / "mist.txt")
+    print("run get_PAR_MIST_hip.x", flush=True)
+    r3 = subprocess.run([os.path.join(root, "parent_amd", "bin", "get_PAR_MIST_hip.x"), theirs, txt],
+                        capture_output=True, text=True, timeout=120)
+    assert r3.returncode == 0, r3.stderr
+    eo, mio, lo, to = O.mist(nD, s1o, s2o)
+    ref_txt = str(tmp_path / "mist_oracle.txt")
+    pa.write_mist_text(ref_txt, eo, mio, lo, to)
+    assert open(txt).read() == open(ref_txt).read()
+
+
+def test_bad_inputs_raise(engine):
+    pa = _pa()
+    vals, nD = _case(10, 500, 2)
+    bad = vals.copy()
+    bad[7, 3] = -0.5  # a negative bond (PARENT.cpp:522-525)
+    engine.set_trajectory(bad)
+    with pytest.raises(pa.ParentError) as ei:
+        engine.prepare([20] * 6)
+    assert ei.value.code == 4
+    bad = vals.copy()
+    bad[3, 3 * nD + 2] = np.float32(2 * np.pi)  # float32(2pi) > 2pi + 5e-9 (modfit histogram)
+    engine.set_trajectory(bad)
+    with pytest.raises(pa.ParentError) as ei:
+        engine.prepare([20] * 6)
+    assert ei.value.code == 5
+    engine.set_trajectory(vals)
+    with pytest.raises(pa.ParentError) as ei:
+        engine.prepare([300] * 6)
+    assert ei.value.code == 7
