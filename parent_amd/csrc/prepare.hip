@@ -77,7 +77,9 @@ __global__ __launch_bounds__(256) void k_scan_raw(const T* __restrict__ v, int N
     smin[threadIdx.x] = mn;
     smax[threadIdx.x] = mx;
     __syncthreads();
-    if (fl == 0 && g < D) {
+    // dihedral extrema come from the shifted values (k_scan_dihedral), so only
+    // bonds and angles publish their raw extrema here
+    if (fl == 0 && g < dih0) {
         for (int q = 1; q < 4; q++) {
             const double a = smin[dl + 64 * q], b = smax[dl + 64 * q];
             mn = a < mn ? a : mn;
