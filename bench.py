@@ -78,8 +78,9 @@ def main():
     ap.add_argument("--workload", default="protein_1500x100k_b50", choices=sorted(WORKLOADS))
     ap.add_argument("--method", default="auto", choices=["auto", "plain", "atomic", "mfma"])
     ap.add_argument("--plain-steps", type=int, default=1, help="timed steps of the plain kernel (0: skip)")
-    ap.add_argument("--cpu-sample-atoms", type=int, default=120)
+    ap.add_argument("--cpu-sample-atoms", type=int, default=500)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tune-report", action="store_true", help="print the autotuner's table")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (--gpus {args.gpus} requested); "
@@ -108,6 +109,13 @@ def main():
         eng.prepare(b6)
         eng.pairs(m)
 
+    tune = None
+    if args.tune_report:
+        eng.prepare(b6)
+        tune = eng.autotune()
+        for t in tune:
+            log(f"tune: method {t['method']} variant {t['variant']} tile {t['tile_rows']}x{t['tile_cols']} "
+                f"block {t['block']}: {t['ms']:.3f} ms, {t['updates_per_s']:.3e} updates/s")
     for _ in range(args.warmup):
         step(method)
     eng.sync()
@@ -173,6 +181,8 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    if tune is not None:
+        out["tuning"] = tune
     eng.close()
     print(json.dumps(out), flush=True)
 

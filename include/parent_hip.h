@@ -117,6 +117,16 @@ typedef struct {
     int kernel_variant;   /* tile configuration id chosen for that method */
 } parent_stats;
 
+/* One row of the autotuner's table. */
+typedef struct {
+    int method;            /* PARENT_METHOD_* */
+    int variant;           /* configuration id within the method */
+    int tile_rows, tile_cols, block;
+    double ms;             /* kernel time on the tuning sample */
+    double updates;        /* pair-frame updates in the sample */
+    double updates_per_s;
+} parent_tune_entry;
+
 /* Creates an engine on HIP device `device`.  No counterpart in the
  * reference (MPI_Init + buffers, PARENT.cpp:131-141, 300-440). */
 int parent_engine_create(int device, parent_engine** out);
@@ -173,6 +183,15 @@ int parent_engine_mist(parent_engine* eng, int32_t* edges, double* mi, int32_t* 
  * get_PAR_MIST.cpp:211-327). */
 int parent_engine_set_entropies(parent_engine* eng, int n_dihedrals, const double* s1d,
                                 const double* s2d);
+
+/* Times every pair-kernel configuration that fits the device on a sample of
+ * the current problem (8 runs of 12 row dofs against all their partners) and
+ * keeps the fastest for PARENT_METHOD_AUTO on this (bins, frames, dofs).
+ * PARENT_METHOD_AUTO tunes on first use by itself; this call re-tunes and
+ * returns the table (up to max_entries rows; *n_entries = rows measured).
+ * No counterpart in the reference (its single loop nest has no variants). */
+int parent_engine_autotune(parent_engine* eng, parent_tune_entry* entries, int max_entries,
+                           int* n_entries);
 
 /* Waits for the engine's stream. */
 int parent_engine_sync(parent_engine* eng);

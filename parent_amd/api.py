@@ -60,6 +60,7 @@ def lib() -> C.CDLL:
         "parent_engine_mist": (i32, [p, p, dp, p, dp]),
         "parent_engine_set_entropies": (i32, [p, i32, dp, dp]),
         "parent_engine_sync": (i32, [p]),
+        "parent_engine_autotune": (i32, [p, p, i32, C.POINTER(i32)]),
         "parent_engine_stats": (i32, [p, p]),
         "parent_synth_values": (i32, [i32, i32, u64, p, i32]),
         "parent_synth_write_bat": (i32, [C.c_char_p, i32, i32, u64, i32]),
@@ -71,6 +72,11 @@ def lib() -> C.CDLL:
         f.argtypes = args
     _lib = L
     return L
+
+
+class _TuneEntry(C.Structure):
+    _fields_ = [("method", C.c_int), ("variant", C.c_int), ("tile_rows", C.c_int), ("tile_cols", C.c_int),
+                ("block", C.c_int), ("ms", C.c_double), ("updates", C.c_double), ("updates_per_s", C.c_double)]
 
 
 class _Stats(C.Structure):
@@ -291,6 +297,13 @@ class Engine:
         tot = np.empty(10, dtype=np.float64)
         self._check(lib().parent_engine_mist(self._h, _ptr(edges), _ptr(mi), _ptr(left), _ptr(tot)))
         return edges, mi, left, tot
+
+    def autotune(self) -> list:
+        """Times every pair-kernel configuration on a sample; returns the table."""
+        arr = (_TuneEntry * 64)()
+        n = C.c_int()
+        self._check(lib().parent_engine_autotune(self._h, C.cast(arr, C.c_void_p), 64, C.byref(n)))
+        return [{k: getattr(arr[i], k) for k, _ in _TuneEntry._fields_} for i in range(min(n.value, 64))]
 
     def sync(self) -> None:
         self._check(lib().parent_engine_sync(self._h))

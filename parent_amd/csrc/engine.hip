@@ -74,6 +74,14 @@ struct parent_engine {
 
     TileList tiles;
     parent_stats stats{};
+
+    // autotuner: the fastest pair-kernel configuration per problem key
+    struct Tuned {
+        int b, a, d, N, D;
+        int method, variant;
+    };
+    std::vector<Tuned> tuned;
+    std::vector<parent_tune_entry> last_tune;
 };
 
 namespace {
@@ -163,6 +171,93 @@ int choose_atomic_config(const parent_engine* e) {
 
 int check_ready(parent_engine* e) {
     if (!e->prepared) return fail(e, PARENT_E_INVALID, "parent_engine_prepare has not succeeded");
+    return PARENT_OK;
+}
+
+// Rows sampled by the autotuner: 8 runs of 12 rows spread over the dof range.
+// 12 is a multiple of every tile height, so every configuration covers exactly
+// the same pairs (all (gi, gj > gi) with gi in the sampled rows).
+std::vector<std::pair<int, int>> tune_rows(int D) {
+    std::vector<std::pair<int, int>> rows;
+    for (int k = 0; k < 8; k++) {
+        int r = (int)((k + 0.5) * (D - 1) / 8.0) / 12 * 12;
+        int r1 = std::min(r + 12, D - 1);
+        if (r < r1 && (rows.empty() || rows.back().second <= r)) rows.push_back({r, r1});
+    }
+    return rows;
+}
+
+std::vector<int> tune_tiles(int D, int tr, int tc, const std::vector<std::pair<int, int>>& rows) {
+    std::vector<int> t;
+    for (auto& rr : rows)
+        for (int r0 = rr.first; r0 < rr.second; r0 += tr)
+            for (int c0 = ((r0 + 1) / tc) * tc; c0 < D; c0 += tc) {
+                t.push_back(r0);
+                t.push_back(c0);
+            }
+    return t;
+}
+
+// Times `launch` twice (first run warms caches) with events on the engine stream.
+template <class F>
+int time_launch(parent_engine* e, F launch, double* ms) {
+    for (int rep = 0; rep < 2; rep++) {
+        HIPCHK(e, hipEventRecord(e->ev[4], e->stream));
+        HIPCHK(e, launch());
+        HIPCHK(e, hipEventRecord(e->ev[5], e->stream));
+        HIPCHK(e, hipStreamSynchronize(e->stream));
+    }
+    *ms = elapsed(e->ev[4], e->ev[5]);
+    return PARENT_OK;
+}
+
+int run_autotune(parent_engine* e) {
+    e->last_tune.clear();
+    const int D = e->D, bmax = bmax2d(e);
+    const auto rows = tune_rows(D);
+    long long npairs = 0;
+    for (auto& rr : rows)
+        for (int r = rr.first; r < rr.second; r++) npairs += D - 1 - r;
+    const double updates = (double)npairs * e->N;
+    PairInputs in = pair_inputs(e);
+    DevBuf dt;
+    int best_m = PARENT_METHOD_PLAIN, best_v = 0;
+    double best_rate = -1;
+    for (int id = 0; id < atomic_config_count(); id++) {
+        const AtomicConfig c = atomic_config(id);
+        if (atomic_lds_bytes(c, bmax) > (size_t)e->lds_max) continue;
+        std::vector<int> t = tune_tiles(D, c.tr, c.tc, rows);
+        HIPCHK(e, dt.ensure(t.size() * sizeof(int)));
+        HIPCHK(e, hipMemcpyAsync(dt.p, t.data(), t.size() * sizeof(int), hipMemcpyHostToDevice, e->stream));
+        double ms = 0;
+        const int nt = (int)(t.size() / 2);
+        if (int rc = time_launch(e, [&] { return launch_pairs_atomic(in, id, bmax, dt.as<int>(), nt, nullptr, nullptr, nullptr, e->stream); }, &ms))
+            return rc;
+        parent_tune_entry te{PARENT_METHOD_ATOMIC, id, c.tr, c.tc, c.block, ms, updates, updates / (ms * 1e-3)};
+        e->last_tune.push_back(te);
+        if (te.updates_per_s > best_rate) {
+            best_rate = te.updates_per_s;
+            best_m = PARENT_METHOD_ATOMIC;
+            best_v = id;
+        }
+    }
+    dt.release();
+    e->tuned.push_back({e->bins[3], e->bins[4], e->bins[5], e->N, D, best_m, best_v});
+    return PARENT_OK;
+}
+
+// The tuned (method, variant) for the current problem, tuning on first use.
+int tuned_choice(parent_engine* e, int* method, int* variant) {
+    for (auto& t : e->tuned)
+        if (t.b == e->bins[3] && t.a == e->bins[4] && t.d == e->bins[5] && t.N == e->N && t.D == e->D) {
+            *method = t.method;
+            *variant = t.variant;
+            return PARENT_OK;
+        }
+    if (int rc = run_autotune(e)) return rc;
+    const auto& t = e->tuned.back();
+    *method = t.method;
+    *variant = t.variant;
     return PARENT_OK;
 }
 
@@ -319,8 +414,12 @@ int parent_engine_pairs(parent_engine* e, int method) {
         return fail(e, PARENT_E_UNSUPPORTED, "2D histogram does not fit in LDS");
     PairInputs in = pair_inputs(e);
     int used = method, variant = 0;
-    if (method == PARENT_METHOD_AUTO || method == PARENT_METHOD_MFMA) used = PARENT_METHOD_ATOMIC;
-    if (used == PARENT_METHOD_ATOMIC) {
+    if (method == PARENT_METHOD_AUTO) {
+        if (int rc = tuned_choice(e, &used, &variant)) return rc;
+    } else if (method == PARENT_METHOD_MFMA) {
+        used = PARENT_METHOD_ATOMIC;
+    }
+    if (used == PARENT_METHOD_ATOMIC && method != PARENT_METHOD_AUTO) {
         variant = choose_atomic_config(e);
         if (variant < 0) used = PARENT_METHOD_PLAIN;
     }
@@ -543,6 +642,22 @@ int parent_engine_mist(parent_engine* e, int32_t* edges, double* mi, int32_t* le
     totals[PARENT_TOT_S1D_D] = t1[2];
     for (int c = 0; c < 6; c++) totals[PARENT_TOT_MI_BB + c] = cat[c];
     totals[PARENT_TOT_TOTAL] = t1[0] + t1[1] + t1[2] - total_mi;
+    return PARENT_OK;
+}
+
+int parent_engine_autotune(parent_engine* e, parent_tune_entry* entries, int max_entries, int* n_entries) {
+    if (!e) return PARENT_E_INVALID;
+    if (int rc = check_ready(e)) return rc;
+    HIPCHK(e, hipSetDevice(e->device));
+    const long long P = n_pairs(e->nD);
+    HIPCHK(e, e->s2d.ensure((size_t)std::max<long long>(P, 1) * 8));
+    e->tuned.erase(std::remove_if(e->tuned.begin(), e->tuned.end(), [&](const parent_engine::Tuned& t) {
+        return t.b == e->bins[3] && t.a == e->bins[4] && t.d == e->bins[5] && t.N == e->N && t.D == e->D;
+    }), e->tuned.end());
+    if (int rc = run_autotune(e)) return rc;
+    const int n = (int)e->last_tune.size();
+    if (n_entries) *n_entries = n;
+    for (int k = 0; k < n && k < max_entries && entries; k++) entries[k] = e->last_tune[k];
     return PARENT_OK;
 }
 
