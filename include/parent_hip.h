@@ -193,6 +193,14 @@ int parent_engine_set_entropies(parent_engine* eng, int n_dihedrals, const doubl
 int parent_engine_autotune(parent_engine* eng, parent_tune_entry* entries, int max_entries,
                            int* n_entries);
 
+/* Pins the configuration `method` uses when called explicitly (not through
+ * PARENT_METHOD_AUTO): variant in [0, count) or -1 for the default.  A
+ * variant whose histograms do not fit the bins makes the method fall back to
+ * PARENT_METHOD_PLAIN.  For testing every configuration the autotuner may
+ * pick; no counterpart in the reference. */
+int parent_engine_set_variant(parent_engine* eng, int method, int variant);
+int parent_engine_variant_count(parent_engine* eng, int method, int* count);
+
 /* Waits for the engine's stream. */
 int parent_engine_sync(parent_engine* eng);
 

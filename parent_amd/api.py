@@ -61,6 +61,8 @@ def lib() -> C.CDLL:
         "parent_engine_set_entropies": (i32, [p, i32, dp, dp]),
         "parent_engine_sync": (i32, [p]),
         "parent_engine_autotune": (i32, [p, p, i32, C.POINTER(i32)]),
+        "parent_engine_set_variant": (i32, [p, i32, i32]),
+        "parent_engine_variant_count": (i32, [p, i32, C.POINTER(i32)]),
         "parent_engine_stats": (i32, [p, p]),
         "parent_synth_values": (i32, [i32, i32, u64, p, i32]),
         "parent_synth_write_bat": (i32, [C.c_char_p, i32, i32, u64, i32]),
@@ -304,6 +306,15 @@ class Engine:
         n = C.c_int()
         self._check(lib().parent_engine_autotune(self._h, C.cast(arr, C.c_void_p), 64, C.byref(n)))
         return [{k: getattr(arr[i], k) for k, _ in _TuneEntry._fields_} for i in range(min(n.value, 64))]
+
+    def set_variant(self, method: int, variant: int) -> None:
+        """Pins the configuration of `method` (-1: default)."""
+        self._check(lib().parent_engine_set_variant(self._h, method, variant))
+
+    def variant_count(self, method: int) -> int:
+        n = C.c_int()
+        self._check(lib().parent_engine_variant_count(self._h, method, C.byref(n)))
+        return n.value
 
     def sync(self) -> None:
         self._check(lib().parent_engine_sync(self._h))

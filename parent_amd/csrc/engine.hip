@@ -82,6 +82,7 @@ struct parent_engine {
     };
     std::vector<Tuned> tuned;
     std::vector<parent_tune_entry> last_tune;
+    int forced_atomic = -1;  // parent_engine_set_variant
 };
 
 namespace {
@@ -162,10 +163,12 @@ int ensure_tiles(parent_engine* e, int config) {
 // Largest tile configuration whose histograms fit the LDS of one workgroup.
 int choose_atomic_config(const parent_engine* e) {
     const int bmax = bmax2d(e);
-    // preference order: measured best first (see the tuning table in the design notes)
-    static const int order[] = {3, 9, 7, 2, 1, 6, 0};
+    if (e->forced_atomic >= 0) return atomic_fits(e->forced_atomic, bmax, (size_t)e->lds_max) ? e->forced_atomic : -1;
+    // fallback order when not autotuning: the best of the autotuner's tables
+    // at 50 bins first, then configurations with larger histogram slots
+    static const int order[] = {2, 3, 1, 9, 0, 11, 13, 10, 15, 16, 14, 18};
     for (int id : order)
-        if (atomic_lds_bytes(atomic_config(id), bmax) <= (size_t)e->lds_max) return id;
+        if (atomic_fits(id, bmax, (size_t)e->lds_max)) return id;
     return -1;
 }
 
@@ -225,13 +228,13 @@ int run_autotune(parent_engine* e) {
     double best_rate = -1;
     for (int id = 0; id < atomic_config_count(); id++) {
         const AtomicConfig c = atomic_config(id);
-        if (atomic_lds_bytes(c, bmax) > (size_t)e->lds_max) continue;
+        if (!atomic_fits(id, bmax, (size_t)e->lds_max)) continue;
         std::vector<int> t = tune_tiles(D, c.tr, c.tc, rows);
         HIPCHK(e, dt.ensure(t.size() * sizeof(int)));
         HIPCHK(e, hipMemcpyAsync(dt.p, t.data(), t.size() * sizeof(int), hipMemcpyHostToDevice, e->stream));
         double ms = 0;
         const int nt = (int)(t.size() / 2);
-        if (int rc = time_launch(e, [&] { return launch_pairs_atomic(in, id, bmax, dt.as<int>(), nt, nullptr, nullptr, nullptr, e->stream); }, &ms))
+        if (int rc = time_launch(e, [&] { return launch_pairs_atomic(in, id, dt.as<int>(), nt, nullptr, nullptr, nullptr, e->stream); }, &ms))
             return rc;
         parent_tune_entry te{PARENT_METHOD_ATOMIC, id, c.tr, c.tc, c.block, ms, updates, updates / (ms * 1e-3)};
         e->last_tune.push_back(te);
@@ -430,7 +433,7 @@ int parent_engine_pairs(parent_engine* e, int method) {
     if (used == PARENT_METHOD_PLAIN) {
         HIPCHK(e, launch_pairs_plain(in, nullptr, e->stream));
     } else if (used == PARENT_METHOD_ATOMIC) {
-        HIPCHK(e, launch_pairs_atomic(in, variant, bmax, e->tiles.dev.as<int>(), e->tiles.ntiles, nullptr, nullptr, nullptr, e->stream));
+        HIPCHK(e, launch_pairs_atomic(in, variant, e->tiles.dev.as<int>(), e->tiles.ntiles, nullptr, nullptr, nullptr, e->stream));
     } else {
         return fail(e, PARENT_E_INVALID, "unknown method");
     }
@@ -500,7 +503,6 @@ int parent_engine_pair_counts(parent_engine* e, int method, int n_pairs_req, con
     DevBuf dpairs, doff, dout;
     HIPCHK(e, dout.ensure(total * 4));
     PairInputs in = pair_inputs(e);
-    const int bmax = bmax2d(e);
     int used = method == PARENT_METHOD_AUTO || method == PARENT_METHOD_MFMA ? PARENT_METHOD_ATOMIC : method;
     int variant = used == PARENT_METHOD_ATOMIC ? choose_atomic_config(e) : 0;
     if (used == PARENT_METHOD_ATOMIC && variant < 0) used = PARENT_METHOD_PLAIN;
@@ -552,7 +554,7 @@ int parent_engine_pair_counts(parent_engine* e, int method, int n_pairs_req, con
         HIPCHK(e, hipMemcpyAsync(dtiles.p, tl.data(), tl.size() * 4, hipMemcpyHostToDevice, e->stream));
         HIPCHK(e, hipMemcpyAsync(dbegin.p, begin.data(), begin.size() * 4, hipMemcpyHostToDevice, e->stream));
         HIPCHK(e, hipMemcpyAsync(dent.p, entries.data(), entries.size() * 4, hipMemcpyHostToDevice, e->stream));
-        HIPCHK(e, launch_pairs_atomic(in, variant, bmax, dtiles.as<int>(), (int)(tl.size() / 2), dbegin.as<int>(), dent.as<int>(), dout.as<int>(), e->stream));
+        HIPCHK(e, launch_pairs_atomic(in, variant, dtiles.as<int>(), (int)(tl.size() / 2), dbegin.as<int>(), dent.as<int>(), dout.as<int>(), e->stream));
         HIPCHK(e, hipStreamSynchronize(e->stream));
         dtiles.release();
         dbegin.release();
@@ -658,6 +660,20 @@ int parent_engine_autotune(parent_engine* e, parent_tune_entry* entries, int max
     const int n = (int)e->last_tune.size();
     if (n_entries) *n_entries = n;
     for (int k = 0; k < n && k < max_entries && entries; k++) entries[k] = e->last_tune[k];
+    return PARENT_OK;
+}
+
+int parent_engine_set_variant(parent_engine* e, int method, int variant) {
+    if (!e) return PARENT_E_INVALID;
+    if (method != PARENT_METHOD_ATOMIC) return fail(e, PARENT_E_INVALID, "only PARENT_METHOD_ATOMIC has variants");
+    if (variant < -1 || variant >= atomic_config_count()) return fail(e, PARENT_E_INVALID, "variant out of range");
+    e->forced_atomic = variant;
+    return PARENT_OK;
+}
+
+int parent_engine_variant_count(parent_engine* e, int method, int* count) {
+    if (!e || !count) return PARENT_E_INVALID;
+    *count = method == PARENT_METHOD_ATOMIC ? atomic_config_count() : (method == PARENT_METHOD_PLAIN ? 1 : 0);
     return PARENT_OK;
 }
 

@@ -51,10 +51,11 @@ struct AtomicConfig {
 };
 int atomic_config_count();
 AtomicConfig atomic_config(int id);
-size_t atomic_lds_bytes(const AtomicConfig& c, int bmax);
-hipError_t launch_pairs_atomic(const PairInputs& in, int config_id, int bmax, const int* tiles,
-                               int ntiles, const int* dump_begin, const int* dump_entries,
-                               int* dump_out, hipStream_t s);
+size_t atomic_lds_bytes(int id);
+bool atomic_fits(int id, int bmax, size_t lds_max);  // histograms of bmax x bmax fit its slots and the LDS
+hipError_t launch_pairs_atomic(const PairInputs& in, int config_id, const int* tiles, int ntiles,
+                               const int* dump_begin, const int* dump_entries, int* dump_out,
+                               hipStream_t s);
 
 // device Prim (MIST)
 struct MistArgs {

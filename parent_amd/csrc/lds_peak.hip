@@ -7,6 +7,11 @@
 //   random_2500    uniform random words of a 2,500-word table (a 50 x 50
 //                  histogram), the pattern of the pair kernels at 50 bins
 //   same_address   all lanes of a wave hit one word
+//   stride_N       lane l hits word l*N (bank structure: N = 32 puts every
+//                  lane of a 32-bank LDS in one bank)
+//   lane_sliced    lane l only touches words w with w % 32 == l % 32 (random
+//                  word otherwise); lanes l and l+32 share words, adding to
+//                  different bytes -- the layout of per-lane sub-histograms
 // Prints one JSON line: atomics/s and atomics per clock per CU per pattern.
 #include <hip/hip_runtime.h>
 
@@ -39,11 +44,17 @@ __global__ __launch_bounds__(kBlock) void k_lds(unsigned* out, int iters) {
     for (int k = 0; k < kAddr; k++) {
         if (MODE == 0) addr[k] = ((wave * kAddr + k) * 64 + lane) % kWords;
         else if (MODE == 1) addr[k] = hash32(blockIdx.x * 7919u + threadIdx.x * 131u + k) % 2500u;
-        else addr[k] = (wave * kAddr + k) % kWords;
+        else if (MODE == 2) addr[k] = (wave * kAddr + k) % kWords;
+        else if (MODE == 3) addr[k] = (lane * 2) % kWords;
+        else if (MODE == 4) addr[k] = (lane * 16) % kWords;
+        else if (MODE == 5) addr[k] = (lane * 32) % kWords;
+        else if (MODE == 6) addr[k] = (lane * 64) % kWords;
+        else addr[k] = ((hash32(blockIdx.x * 7919u + threadIdx.x * 131u + k) % 320u) * 32u + (lane & 31)) % kWords;
     }
+    const unsigned inc = MODE == 7 ? (1u << (8 * ((lane >> 5) & 1))) : 1u;
     for (int it = 0; it < iters; it++) {
 #pragma unroll
-        for (int k = 0; k < kAddr; k++) atomicAdd(&h[addr[k]], 1u);
+        for (int k = 0; k < kAddr; k++) atomicAdd(&h[addr[k]], inc);
     }
     __syncthreads();
     unsigned s = 0;
@@ -83,12 +94,19 @@ int main() {
     const double r0 = run<0>(blocks, 2048, dout);
     const double r1 = run<1>(blocks, 2048, dout);
     const double r2 = run<2>(blocks, 256, dout);
+    const double r3 = run<3>(blocks, 1024, dout);
+    const double r4 = run<4>(blocks, 512, dout);
+    const double r5 = run<5>(blocks, 256, dout);
+    const double r6 = run<6>(blocks, 256, dout);
+    const double r7 = run<7>(blocks, 2048, dout);
     auto per = [&](double r) { return r / (cus * clk); };
-    printf("{\"device\": \"%s\", \"cus\": %d, \"clock_mhz\": %.0f, "
-           "\"conflict_free\": {\"atomics_per_s\": %.4e, \"per_clk_per_cu\": %.2f}, "
-           "\"random_2500\": {\"atomics_per_s\": %.4e, \"per_clk_per_cu\": %.2f}, "
-           "\"same_address\": {\"atomics_per_s\": %.4e, \"per_clk_per_cu\": %.2f}}\n",
-           p.gcnArchName, cus, clk / 1e6, r0, per(r0), r1, per(r1), r2, per(r2));
+    printf("{\"device\": \"%s\", \"cus\": %d, \"clock_mhz\": %.0f", p.gcnArchName, cus, clk / 1e6);
+    const char* names[] = {"conflict_free", "random_2500", "same_address", "stride_2", "stride_16",
+                           "stride_32", "stride_64", "lane_sliced"};
+    const double rates[] = {r0, r1, r2, r3, r4, r5, r6, r7};
+    for (int k = 0; k < 8; k++)
+        printf(", \"%s\": {\"atomics_per_s\": %.4e, \"per_clk_per_cu\": %.2f}", names[k], rates[k], per(rates[k]));
+    printf("}\n");
     (void)hipFree(dout);
     return 0;
 }

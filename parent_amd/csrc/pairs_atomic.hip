@@ -113,6 +113,9 @@ __global__ __launch_bounds__(256) void k_pairs_plain(PairInputs in, long long p0
 hipError_t launch_pairs_plain(const PairInputs& in, const CountsDump* dump, hipStream_t s) {
     const int bmax = max(in.B.b, max(in.B.a, in.B.d));
     const size_t lds = (size_t)bmax * bmax * sizeof(unsigned);
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_pairs_plain),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
     const long long total = dump ? dump->n : (long long)in.D * (in.D - 1) / 2;
     const long long chunk = 1LL << 23;
     for (long long p0 = 0; p0 < total; p0 += chunk) {
@@ -131,10 +134,17 @@ hipError_t launch_pairs_plain(const PairInputs& in, const CountsDump* dump, hipS
 // takes tile (b % 8) * chunk + b / 8 of its range, so the workgroups that the
 // dispatcher places on one XCD (round robin over 8 XCDs) walk one contiguous
 // run of tiles and share its row-dof index rows in that XCD's L2.
-template <int TR, int TC, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_pairs_atomic(PairInputs in, int bmax,
-                                                        const int* __restrict__ tiles, int t_begin,
-                                                        int t_end, const int* __restrict__ dump_begin,
+//
+// SLOTW is the compile-time word stride between the tile's histograms
+// (>= bins_i * bins_j), so a histogram's base folds into the ds_add_u32
+// immediate offset.  Tiles whose TR*TC pairs are all valid and whose columns
+// share one bin count (every tile off the diagonal band when the 2D bin counts
+// are equal) take a branch-free fill: per frame one 24-bit multiply per row
+// and one add + one LDS atomic per pair.
+template <int TR, int TC, int BLOCK, int SLOTW>
+__global__ __launch_bounds__(BLOCK) void k_pairs_atomic(PairInputs in, const int* __restrict__ tiles,
+                                                        int t_begin, int t_end,
+                                                        const int* __restrict__ dump_begin,
                                                         const int* __restrict__ dump_entries,
                                                         int* __restrict__ dump_out) {
     extern __shared__ unsigned hist[];
@@ -143,7 +153,6 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_atomic(PairInputs in, int bmax,
     if (t >= t_end || (int)(blockIdx.x / 8) >= chunk) return;
     const int r0 = tiles[2 * t], c0 = tiles[2 * t + 1];
     const int D = in.D, N = in.N, nD = in.nD;
-    const int slot = bmax * bmax;
 
     int gr[TR], gc[TC], Bc[TC];
 #pragma unroll
@@ -154,43 +163,72 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_atomic(PairInputs in, int bmax,
         Bc[c] = gc[c] < D ? in.B.of(dof_type(nD, gc[c])) : 1;
     }
     unsigned valid = 0;  // bit r*TC+c
+    bool uniform = true;
 #pragma unroll
     for (int r = 0; r < TR; r++)
 #pragma unroll
         for (int c = 0; c < TC; c++)
             if (gr[r] < D && gc[c] < D && gr[r] < gc[c]) valid |= 1u << (r * TC + c);
+#pragma unroll
+    for (int c = 1; c < TC; c++) uniform = uniform && Bc[c] == Bc[0];
+    const bool fast = uniform && valid == (1u << (TR * TC)) - 1u;
 
-    for (int i = threadIdx.x; i < TR * TC * slot; i += BLOCK) hist[i] = 0;
+    for (int i = threadIdx.x; i < TR * TC * SLOTW; i += BLOCK) hist[i] = 0;
     __syncthreads();
 
     const int per_iter = BLOCK * 16;
     const int niter = (N + per_iter - 1) / per_iter;
     const int nfull = N / per_iter;
-    for (int it = 0; it < niter; it++) {
+    const uint8_t* rows[TR];
+    const uint8_t* cols[TC];
+#pragma unroll
+    for (int r = 0; r < TR; r++) rows[r] = in.bins + (size_t)(gr[r] < D ? gr[r] : 0) * in.Np;
+#pragma unroll
+    for (int c = 0; c < TC; c++) cols[c] = in.bins + (size_t)(gc[c] < D ? gc[c] : 0) * in.Np;
+
+    int it0 = 0;
+    if (fast) {
+        const unsigned B = (unsigned)Bc[0];
+        for (int it = 0; it < nfull; it++) {
+            const int f = it * per_iter + threadIdx.x * 16;
+            uint4 R[TR], C[TC];
+#pragma unroll
+            for (int r = 0; r < TR; r++) R[r] = *reinterpret_cast<const uint4*>(rows[r] + f);
+#pragma unroll
+            for (int c = 0; c < TC; c++) C[c] = *reinterpret_cast<const uint4*>(cols[c] + f);
+#pragma unroll
+            for (int e = 0; e < 16; e++) {
+                unsigned ro[TR];
+#pragma unroll
+                for (int r = 0; r < TR; r++) ro[r] = __umul24(byte_of(R[r], e), B);
+#pragma unroll
+                for (int r = 0; r < TR; r++)
+#pragma unroll
+                    for (int c = 0; c < TC; c++) atomicAdd(&hist[(r * TC + c) * SLOTW + ro[r] + byte_of(C[c], e)], 1u);
+            }
+        }
+        it0 = nfull;
+    }
+    // general fill: diagonal-band / edge tiles, mixed bin counts, and the
+    // final partial iteration
+    for (int it = it0; it < niter; it++) {
         const int f = it * per_iter + threadIdx.x * 16;
         uint4 R[TR], C[TC];
 #pragma unroll
         for (int r = 0; r < TR; r++)
-            R[r] = (gr[r] < D && f < in.Np)
-                       ? *reinterpret_cast<const uint4*>(in.bins + (size_t)gr[r] * in.Np + f)
-                       : make_uint4(0, 0, 0, 0);
+            R[r] = f < in.Np ? *reinterpret_cast<const uint4*>(rows[r] + f) : make_uint4(0, 0, 0, 0);
 #pragma unroll
         for (int c = 0; c < TC; c++)
-            C[c] = (gc[c] < D && f < in.Np)
-                       ? *reinterpret_cast<const uint4*>(in.bins + (size_t)gc[c] * in.Np + f)
-                       : make_uint4(0, 0, 0, 0);
-        const int nvalid = it < nfull ? 16 : min(16, max(0, N - f));
+            C[c] = f < in.Np ? *reinterpret_cast<const uint4*>(cols[c] + f) : make_uint4(0, 0, 0, 0);
+        const int nvalid = min(16, max(0, N - f));
+        for (int e = 0; e < nvalid; e++) {
 #pragma unroll
-        for (int e = 0; e < 16; e++) {
-            if (e < nvalid) {
+            for (int r = 0; r < TR; r++) {
+                const unsigned a = byte_of(R[r], e);
 #pragma unroll
-                for (int r = 0; r < TR; r++) {
-                    const unsigned a = byte_of(R[r], e);
-#pragma unroll
-                    for (int c = 0; c < TC; c++) {
-                        if (valid & (1u << (r * TC + c)))
-                            atomicAdd(&hist[(r * TC + c) * slot + a * Bc[c] + byte_of(C[c], e)], 1u);
-                    }
+                for (int c = 0; c < TC; c++) {
+                    if (valid & (1u << (r * TC + c)))
+                        atomicAdd(&hist[(r * TC + c) * SLOTW + __umul24(a, (unsigned)Bc[c]) + byte_of(C[c], e)], 1u);
                 }
             }
         }
@@ -205,7 +243,7 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_atomic(PairInputs in, int bmax,
             const long long off = ((long long)dump_entries[3 * e + 1] << 31) | (long long)dump_entries[3 * e + 2];
             const int r = sl / TC, c = sl % TC;
             const int cells = in.B.of(dof_type(nD, gr[r])) * Bc[c];
-            for (int i = threadIdx.x; i < cells; i += BLOCK) dump_out[off + i] = (int)hist[sl * slot + i];
+            for (int i = threadIdx.x; i < cells; i += BLOCK) dump_out[off + i] = (int)hist[sl * SLOTW + i];
         }
         return;
     }
@@ -217,7 +255,7 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_atomic(PairInputs in, int bmax,
         const int r = sl / TC, c = sl % TC;
         const int gi = gr[r], gj = gc[c];
         const int cells = in.B.of(dof_type(nD, gi)) * Bc[c];
-        const unsigned* h = hist + sl * slot;
+        const unsigned* h = hist + sl * SLOTW;
         double T = 0.0;
         int occ = 0;
         for (int i = lane; i < cells; i += 64) {
@@ -237,41 +275,57 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_atomic(PairInputs in, int bmax,
 
 namespace {
 
+// (tile rows, tile columns, threads, slot words); slot words bound the bin
+// count: 2560 -> B <= 50, 5632 -> B <= 75, 10240 -> B <= 101, 40960 -> B <= 202.
+#define PARENT_ATOMIC_VARIANTS(X)     \
+    X(0, 1, 1, 256, 2560)             \
+    X(1, 2, 2, 256, 2560)             \
+    X(2, 2, 2, 512, 2560)             \
+    X(3, 2, 4, 512, 2560)             \
+    X(4, 1, 4, 256, 2560)             \
+    X(5, 1, 8, 512, 2560)             \
+    X(6, 3, 3, 512, 2560)             \
+    X(7, 4, 4, 1024, 2560)            \
+    X(8, 2, 2, 1024, 2560)            \
+    X(9, 1, 2, 256, 2560)             \
+    X(10, 1, 1, 256, 5632)            \
+    X(11, 2, 2, 512, 5632)            \
+    X(12, 1, 4, 256, 5632)            \
+    X(13, 2, 3, 512, 5632)            \
+    X(14, 1, 1, 256, 10240)           \
+    X(15, 2, 2, 512, 10240)           \
+    X(16, 1, 2, 256, 10240)           \
+    X(17, 1, 3, 512, 10240)           \
+    X(18, 1, 1, 256, 40960)
+
 struct AtomicVariant {
     AtomicConfig cfg;
+    int slotw;
     const void* fn;
 };
 
-#define PARENT_ATOMIC_VARIANT(TR, TC, BLK) \
-    AtomicVariant{AtomicConfig{TR, TC, BLK}, reinterpret_cast<const void*>(&k_pairs_atomic<TR, TC, BLK>)}
-
-const AtomicVariant kAtomicVariants[] = {
-    PARENT_ATOMIC_VARIANT(1, 1, 256),  PARENT_ATOMIC_VARIANT(2, 2, 256),
-    PARENT_ATOMIC_VARIANT(2, 2, 512),  PARENT_ATOMIC_VARIANT(3, 3, 512),
-    PARENT_ATOMIC_VARIANT(3, 3, 1024), PARENT_ATOMIC_VARIANT(4, 4, 1024),
-    PARENT_ATOMIC_VARIANT(1, 4, 256),  PARENT_ATOMIC_VARIANT(2, 4, 512),
-    PARENT_ATOMIC_VARIANT(1, 8, 512),  PARENT_ATOMIC_VARIANT(4, 4, 512),
-};
-
-template <int TR, int TC, int BLK>
-void launch_one(dim3 grid, size_t lds, hipStream_t s, const PairInputs& in, int bmax, const int* tiles,
-                int t0, int t1, const int* db, const int* de, int* dout) {
-    k_pairs_atomic<TR, TC, BLK><<<grid, BLK, lds, s>>>(in, bmax, tiles, t0, t1, db, de, dout);
-}
+#define PARENT_ATOMIC_ENTRY(ID, TR, TC, BLK, SW) \
+    AtomicVariant{AtomicConfig{TR, TC, BLK}, SW, reinterpret_cast<const void*>(&k_pairs_atomic<TR, TC, BLK, SW>)},
+const AtomicVariant kAtomicVariants[] = {PARENT_ATOMIC_VARIANTS(PARENT_ATOMIC_ENTRY)};
+#undef PARENT_ATOMIC_ENTRY
 
 }  // namespace
 
 int atomic_config_count() { return (int)(sizeof(kAtomicVariants) / sizeof(kAtomicVariants[0])); }
 AtomicConfig atomic_config(int id) { return kAtomicVariants[id].cfg; }
-size_t atomic_lds_bytes(const AtomicConfig& c, int bmax) {
-    return (size_t)c.tr * c.tc * bmax * bmax * sizeof(unsigned);
+size_t atomic_lds_bytes(int id) {
+    const AtomicVariant& v = kAtomicVariants[id];
+    return (size_t)v.cfg.tr * v.cfg.tc * v.slotw * sizeof(unsigned);
+}
+bool atomic_fits(int id, int bmax, size_t lds_max) {
+    return (long long)bmax * bmax <= kAtomicVariants[id].slotw && atomic_lds_bytes(id) <= lds_max;
 }
 
-hipError_t launch_pairs_atomic(const PairInputs& in, int id, int bmax, const int* tiles, int ntiles,
+hipError_t launch_pairs_atomic(const PairInputs& in, int id, const int* tiles, int ntiles,
                                const int* dump_begin, const int* dump_entries, int* dump_out,
                                hipStream_t s) {
-    const AtomicConfig c = kAtomicVariants[id].cfg;
-    const size_t lds = atomic_lds_bytes(c, bmax);
+    if (id < 0 || id >= atomic_config_count()) return hipErrorInvalidValue;
+    const size_t lds = atomic_lds_bytes(id);
     hipError_t e = hipFuncSetAttribute(kAtomicVariants[id].fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     const int per_launch = 1 << 22;
@@ -280,19 +334,13 @@ hipError_t launch_pairs_atomic(const PairInputs& in, int id, int bmax, const int
         const int chunk = (t1 - t0 + 7) / 8;
         dim3 grid((unsigned)chunk * 8);
         switch (id) {
-#define CASE(I, TR, TC, BLK) \
-    case I: launch_one<TR, TC, BLK>(grid, lds, s, in, bmax, tiles, t0, t1, dump_begin, dump_entries, dump_out); break;
-            CASE(0, 1, 1, 256)
-            CASE(1, 2, 2, 256)
-            CASE(2, 2, 2, 512)
-            CASE(3, 3, 3, 512)
-            CASE(4, 3, 3, 1024)
-            CASE(5, 4, 4, 1024)
-            CASE(6, 1, 4, 256)
-            CASE(7, 2, 4, 512)
-            CASE(8, 1, 8, 512)
-            CASE(9, 4, 4, 512)
-#undef CASE
+#define PARENT_ATOMIC_CASE(ID, TR, TC, BLK, SW)                                                  \
+    case ID:                                                                                      \
+        k_pairs_atomic<TR, TC, BLK, SW><<<grid, BLK, lds, s>>>(in, tiles, t0, t1, dump_begin,    \
+                                                               dump_entries, dump_out);           \
+        break;
+            PARENT_ATOMIC_VARIANTS(PARENT_ATOMIC_CASE)
+#undef PARENT_ATOMIC_CASE
             default: return hipErrorInvalidValue;
         }
     }

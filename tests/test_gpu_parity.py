@@ -132,6 +132,40 @@ def test_entropies_vs_oracle(engine, method, atoms, frames, seed, bins):
           f"max|dS2D| {np.max(np.abs(s2 - s2o)):.2e} max|dMI| {err.max():.2e}", flush=True)
 
 
+def test_every_atomic_variant_bit_exact(engine):
+    """Each tile configuration the autotuner can pick: counts bit-exact on a pair
+    sample, S2D and MI within tolerance on all pairs (fast and general fills)."""
+    pa = _pa()
+    for bins, atoms, frames in (([50] * 6, 26, 4097), ([10, 10, 10, 20, 30, 40], 22, 3000),
+                                ([75] * 6, 14, 2500), ([150] * 6, 10, 1500)):
+        vals, nD = _case(atoms, frames, 13)
+        engine.set_trajectory(vals)
+        engine.prepare(bins)
+        s1o, s2o, mno, mxo, _, shifted = _oracle_full(vals, nD, bins)
+        D = 3 * nD + 3
+        pairs = [(0, 1), (0, D - 1), (5, 6), (nD + 1, nD + 3), (2 * nD + 2, 2 * nD + 3), (D - 2, D - 1), (3, 2 * nD + 5)]
+        ref = []
+        for gi, gj in pairs:
+            ti, tj = O.type_of(nD, gi), O.type_of(nD, gj)
+            ref.append(O.entropy2d(ti, shifted[gi], mno[gi], mxo[gi], bins[3 + ti],
+                                   tj, shifted[gj], mno[gj], mxo[gj], bins[3 + tj], counts=True)[1])
+        tested = 0
+        for v in range(engine.variant_count(pa.METHOD_ATOMIC)):
+            engine.set_variant(pa.METHOD_ATOMIC, v)
+            engine.pairs(pa.METHOD_ATOMIC)
+            st = engine.stats()
+            if st["method"] != pa.METHOD_ATOMIC:
+                continue  # histograms do not fit this variant's slots
+            tested += 1
+            print(f"  bins={bins} variant {v}", flush=True)
+            for h, ho in zip(engine.pair_counts(pairs, pa.METHOD_ATOMIC, bins[3:]), ref):
+                assert np.array_equal(h.astype(np.int64), ho), f"variant {v}"
+            s1, s2 = engine.s1d(), engine.s2d()
+            assert np.max(np.abs(s2 - s2o)) <= S_ABS, f"variant {v}"
+        engine.set_variant(pa.METHOD_ATOMIC, -1)
+        assert tested >= 1
+
+
 @pytest.mark.parametrize("atoms,frames,seed", [(12, 1000, 3), (40, 3000, 21)])
 def test_mist_identical(engine, atoms, frames, seed):
     print(f"mist atoms={atoms} frames={frames}", flush=True)
