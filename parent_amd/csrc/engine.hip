@@ -133,16 +133,24 @@ PairInputs pair_inputs(parent_engine* e) {
 
 // Tiles of a TR x TC configuration over the upper triangle: row blocks at
 // multiples of TR, column blocks at multiples of TC, kept when some column of
-// the block lies above the block's first row.
+// the block lies above the block's first row.  Order: bands of ~16 rows, and
+// within a band column by column, so the workgroups running at the same time
+// (one contiguous run of the list per XCD) read the same column-dof index rows
+// from L2 while their band's row-dof rows stay resident there.
+void push_band(std::vector<int>& t, int D, int tr, int tc, int b0, int b1) {
+    const int cfirst = ((b0 + 1) / tc) * tc;
+    for (int c0 = cfirst; c0 < D; c0 += tc)
+        for (int r0 = b0; r0 < b1; r0 += tr)
+            if (c0 + tc - 1 > r0) {
+                t.push_back(r0);
+                t.push_back(c0);
+            }
+}
+
 std::vector<int> make_tiles(int D, int tr, int tc) {
     std::vector<int> t;
-    for (int r0 = 0; r0 < D - 1; r0 += tr) {
-        int c0 = ((r0 + 1) / tc) * tc;
-        for (; c0 < D; c0 += tc) {
-            t.push_back(r0);
-            t.push_back(c0);
-        }
-    }
+    const int band = std::max(tr, 16 / tr * tr);
+    for (int b0 = 0; b0 < D - 1; b0 += band) push_band(t, D, tr, tc, b0, std::min(b0 + band, D - 1));
     return t;
 }
 
@@ -192,12 +200,7 @@ std::vector<std::pair<int, int>> tune_rows(int D) {
 
 std::vector<int> tune_tiles(int D, int tr, int tc, const std::vector<std::pair<int, int>>& rows) {
     std::vector<int> t;
-    for (auto& rr : rows)
-        for (int r0 = rr.first; r0 < rr.second; r0 += tr)
-            for (int c0 = ((r0 + 1) / tc) * tc; c0 < D; c0 += tc) {
-                t.push_back(r0);
-                t.push_back(c0);
-            }
+    for (auto& rr : rows) push_band(t, D, tr, tc, rr.first, rr.second);
     return t;
 }
 

@@ -273,39 +273,242 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_atomic(PairInputs in, const int
     }
 }
 
+// ---------------------------------------------------------------------------
+// sliced: bank-conflict-free LDS histograms
+// ---------------------------------------------------------------------------
+// The LDS serves ds_add_u32 16 lanes per clock over 32 banks; random cells
+// from 16 lanes collide in banks and cost ~half the peak rate.  Here every
+// cell owns 16 u32 sub-counters, one per lane position within a 16-lane
+// group: word(cell, s) = cell * 16 + s with s = lane % 16, whose bank is
+// s + 16 * (cell & 1) -- distinct for the 16 lanes served together whatever
+// their cells.  Lanes of different groups share words, which the hardware
+// serves in different cycles.  The count of a cell is the sum of its 16
+// sub-counters (epilogue).  64 bytes per cell: one 50 x 50 histogram fills
+// the 160 KB of LDS.
+template <int TR, int TC, int BLOCK, int SLOTC>
+__global__ __launch_bounds__(BLOCK) void k_pairs_sliced(PairInputs in, const int* __restrict__ tiles,
+                                                        int t_begin, int t_end,
+                                                        const int* __restrict__ dump_begin,
+                                                        const int* __restrict__ dump_entries,
+                                                        int* __restrict__ dump_out) {
+    extern __shared__ unsigned hist[];
+    constexpr int kSlotW = SLOTC * 16;
+    const int chunk = (t_end - t_begin + 7) / 8;
+    const int t = t_begin + (int)(blockIdx.x % 8) * chunk + (int)(blockIdx.x / 8);
+    if (t >= t_end || (int)(blockIdx.x / 8) >= chunk) return;
+    const int r0 = tiles[2 * t], c0 = tiles[2 * t + 1];
+    const int D = in.D, N = in.N, nD = in.nD;
+    const unsigned s16 = threadIdx.x & 15;
+
+    int gr[TR], gc[TC], Bc[TC];
+#pragma unroll
+    for (int r = 0; r < TR; r++) gr[r] = r0 + r;
+#pragma unroll
+    for (int c = 0; c < TC; c++) {
+        gc[c] = c0 + c;
+        Bc[c] = gc[c] < D ? in.B.of(dof_type(nD, gc[c])) : 1;
+    }
+    unsigned valid = 0;
+    bool uniform = true;
+#pragma unroll
+    for (int r = 0; r < TR; r++)
+#pragma unroll
+        for (int c = 0; c < TC; c++)
+            if (gr[r] < D && gc[c] < D && gr[r] < gc[c]) valid |= 1u << (r * TC + c);
+#pragma unroll
+    for (int c = 1; c < TC; c++) uniform = uniform && Bc[c] == Bc[0];
+    const bool fast = uniform && valid == (1u << (TR * TC)) - 1u;
+
+    {
+        uint4* h4 = reinterpret_cast<uint4*>(hist);
+        for (int i = threadIdx.x; i < TR * TC * kSlotW / 4; i += BLOCK) h4[i] = make_uint4(0, 0, 0, 0);
+    }
+    __syncthreads();
+
+    const int per_iter = BLOCK * 16;
+    const int niter = (N + per_iter - 1) / per_iter;
+    const int nfull = N / per_iter;
+    const uint8_t* rows[TR];
+    const uint8_t* cols[TC];
+#pragma unroll
+    for (int r = 0; r < TR; r++) rows[r] = in.bins + (size_t)(gr[r] < D ? gr[r] : 0) * in.Np;
+#pragma unroll
+    for (int c = 0; c < TC; c++) cols[c] = in.bins + (size_t)(gc[c] < D ? gc[c] : 0) * in.Np;
+
+    int it0 = 0;
+    if (fast) {
+        const unsigned B16 = (unsigned)Bc[0] * 16u;
+        for (int it = 0; it < nfull; it++) {
+            const int f = it * per_iter + threadIdx.x * 16;
+            uint4 R[TR], C[TC];
+#pragma unroll
+            for (int r = 0; r < TR; r++) R[r] = *reinterpret_cast<const uint4*>(rows[r] + f);
+#pragma unroll
+            for (int c = 0; c < TC; c++) C[c] = *reinterpret_cast<const uint4*>(cols[c] + f);
+#pragma unroll
+            for (int e = 0; e < 16; e++) {
+                unsigned ro[TR], co[TC];
+#pragma unroll
+                for (int r = 0; r < TR; r++) ro[r] = __umul24(byte_of(R[r], e), B16) + s16;
+#pragma unroll
+                for (int c = 0; c < TC; c++) co[c] = byte_of(C[c], e) << 4;
+#pragma unroll
+                for (int r = 0; r < TR; r++)
+#pragma unroll
+                    for (int c = 0; c < TC; c++) atomicAdd(&hist[(r * TC + c) * kSlotW + ro[r] + co[c]], 1u);
+            }
+        }
+        it0 = nfull;
+    }
+    for (int it = it0; it < niter; it++) {
+        const int f = it * per_iter + threadIdx.x * 16;
+        uint4 R[TR], C[TC];
+#pragma unroll
+        for (int r = 0; r < TR; r++)
+            R[r] = f < in.Np ? *reinterpret_cast<const uint4*>(rows[r] + f) : make_uint4(0, 0, 0, 0);
+#pragma unroll
+        for (int c = 0; c < TC; c++)
+            C[c] = f < in.Np ? *reinterpret_cast<const uint4*>(cols[c] + f) : make_uint4(0, 0, 0, 0);
+        const int nvalid = min(16, max(0, N - f));
+        for (int e = 0; e < nvalid; e++) {
+#pragma unroll
+            for (int r = 0; r < TR; r++) {
+                const unsigned a = byte_of(R[r], e);
+#pragma unroll
+                for (int c = 0; c < TC; c++) {
+                    if (valid & (1u << (r * TC + c)))
+                        atomicAdd(&hist[(r * TC + c) * kSlotW + ((__umul24(a, (unsigned)Bc[c]) + byte_of(C[c], e)) << 4) + s16], 1u);
+                }
+            }
+        }
+    }
+    __syncthreads();
+
+    // count of cell i of slot sl: sum of its 16 sub-counters
+    auto cell_count = [&](int sl, int i) {
+        const uint4* q = reinterpret_cast<const uint4*>(hist + sl * kSlotW + i * 16);
+        unsigned n = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const uint4 v = q[k];
+            n += v.x + v.y + v.z + v.w;
+        }
+        return n;
+    };
+    if (dump_out) {
+        const int e0 = dump_begin[t], e1 = dump_begin[t + 1];
+        for (int e = e0; e < e1; e++) {
+            const int sl = dump_entries[3 * e];
+            const long long off = ((long long)dump_entries[3 * e + 1] << 31) | (long long)dump_entries[3 * e + 2];
+            const int r = sl / TC, c = sl % TC;
+            const int cells = in.B.of(dof_type(nD, gr[r])) * Bc[c];
+            for (int i = threadIdx.x; i < cells; i += BLOCK) dump_out[off + i] = (int)cell_count(sl, i);
+        }
+        return;
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    constexpr int kWaves = BLOCK / 64;
+    for (int sl = 0; sl < TR * TC; sl++) {
+        if (!(valid & (1u << sl))) continue;
+        const int r = sl / TC, c = sl % TC;
+        const int gi = gr[r], gj = gc[c];
+        const int cells = in.B.of(dof_type(nD, gi)) * Bc[c];
+        // all waves share a slot: per-wave partial sums, then wave 0 combines
+        __shared__ double sT[kWaves];
+        __shared__ int sOcc[kWaves];
+        double T = 0.0;
+        int occ = 0;
+        for (int i = threadIdx.x; i < cells; i += BLOCK) {
+            const unsigned n = cell_count(sl, i);
+            if (n) {
+                T += in.nlogn[n];
+                occ++;
+            }
+        }
+        T = wave_sum(T);
+        occ = wave_sum(occ);
+        if (lane == 0) {
+            sT[wid] = T;
+            sOcc[wid] = occ;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double tt = 0.0;
+            int oo = 0;
+            for (int w = 0; w < kWaves; w++) {
+                tt += sT[w];
+                oo += sOcc[w];
+            }
+            in.s2d[par_offset(nD, gi, gj)] = ((in.logN + in.c2[gi]) + in.c2[gj] + (oo - 1.0) / (2.0 * N)) - tt / N;
+        }
+        __syncthreads();
+    }
+}
+
 namespace {
 
-// (tile rows, tile columns, threads, slot words); slot words bound the bin
-// count: 2560 -> B <= 50, 5632 -> B <= 75, 10240 -> B <= 101, 40960 -> B <= 202.
+// (kind, tile rows, tile columns, threads, slot size).  kind 0: plain LDS
+// histograms, slot size in words (2560 -> B <= 50, 5632 -> B <= 75,
+// 10240 -> B <= 101, 40960 -> B <= 202); kind 1: sliced histograms, slot size
+// in cells (64 bytes each; 2500 -> B <= 50, 1225 -> B <= 35, 900 -> B <= 30,
+// 400 -> B <= 20, 100 -> B <= 10).
 #define PARENT_ATOMIC_VARIANTS(X)     \
-    X(0, 1, 1, 256, 2560)             \
-    X(1, 2, 2, 256, 2560)             \
-    X(2, 2, 2, 512, 2560)             \
-    X(3, 2, 4, 512, 2560)             \
-    X(4, 1, 4, 256, 2560)             \
-    X(5, 1, 8, 512, 2560)             \
-    X(6, 3, 3, 512, 2560)             \
-    X(7, 4, 4, 1024, 2560)            \
-    X(8, 2, 2, 1024, 2560)            \
-    X(9, 1, 2, 256, 2560)             \
-    X(10, 1, 1, 256, 5632)            \
-    X(11, 2, 2, 512, 5632)            \
-    X(12, 1, 4, 256, 5632)            \
-    X(13, 2, 3, 512, 5632)            \
-    X(14, 1, 1, 256, 10240)           \
-    X(15, 2, 2, 512, 10240)           \
-    X(16, 1, 2, 256, 10240)           \
-    X(17, 1, 3, 512, 10240)           \
-    X(18, 1, 1, 256, 40960)
+    X(0, 0, 1, 1, 256, 2560)          \
+    X(1, 0, 2, 2, 256, 2560)          \
+    X(2, 0, 2, 2, 512, 2560)          \
+    X(3, 0, 2, 4, 512, 2560)          \
+    X(4, 0, 1, 4, 256, 2560)          \
+    X(5, 0, 1, 8, 512, 2560)          \
+    X(6, 0, 3, 3, 512, 2560)          \
+    X(7, 0, 4, 4, 1024, 2560)         \
+    X(8, 0, 2, 2, 1024, 2560)         \
+    X(9, 0, 1, 2, 256, 2560)          \
+    X(10, 0, 1, 1, 256, 5632)         \
+    X(11, 0, 2, 2, 512, 5632)         \
+    X(12, 0, 1, 4, 256, 5632)         \
+    X(13, 0, 2, 3, 512, 5632)         \
+    X(14, 0, 1, 1, 256, 10240)        \
+    X(15, 0, 2, 2, 512, 10240)        \
+    X(16, 0, 1, 2, 256, 10240)        \
+    X(17, 0, 1, 3, 512, 10240)        \
+    X(18, 0, 1, 1, 256, 40960)        \
+    X(19, 1, 1, 1, 1024, 2500)        \
+    X(20, 1, 1, 1, 512, 2500)         \
+    X(21, 1, 1, 1, 1024, 1225)        \
+    X(22, 1, 1, 2, 1024, 1225)        \
+    X(23, 1, 1, 1, 512, 1225)         \
+    X(24, 1, 1, 2, 1024, 900)         \
+    X(25, 1, 1, 1, 512, 900)          \
+    X(26, 1, 2, 2, 1024, 400)         \
+    X(27, 1, 1, 4, 1024, 400)         \
+    X(28, 1, 1, 1, 256, 400)          \
+    X(29, 1, 2, 2, 512, 100)          \
+    X(30, 1, 4, 4, 1024, 100)
 
 struct AtomicVariant {
     AtomicConfig cfg;
-    int slotw;
+    int kind;
+    int slot;  // words (kind 0) or cells (kind 1)
     const void* fn;
 };
 
-#define PARENT_ATOMIC_ENTRY(ID, TR, TC, BLK, SW) \
-    AtomicVariant{AtomicConfig{TR, TC, BLK}, SW, reinterpret_cast<const void*>(&k_pairs_atomic<TR, TC, BLK, SW>)},
+template <int KIND, int TR, int TC, int BLK, int SL>
+const void* atomic_fn() {
+    if constexpr (KIND == 0) return reinterpret_cast<const void*>(&k_pairs_atomic<TR, TC, BLK, SL>);
+    else return reinterpret_cast<const void*>(&k_pairs_sliced<TR, TC, BLK, SL>);
+}
+
+template <int KIND, int TR, int TC, int BLK, int SL>
+void launch_variant(dim3 grid, size_t lds, hipStream_t s, const PairInputs& in, const int* tiles, int t0,
+                    int t1, const int* db, const int* de, int* dout) {
+    if constexpr (KIND == 0)
+        k_pairs_atomic<TR, TC, BLK, SL><<<grid, BLK, lds, s>>>(in, tiles, t0, t1, db, de, dout);
+    else
+        k_pairs_sliced<TR, TC, BLK, SL><<<grid, BLK, lds, s>>>(in, tiles, t0, t1, db, de, dout);
+}
+
+#define PARENT_ATOMIC_ENTRY(ID, K, TR, TC, BLK, SL) \
+    AtomicVariant{AtomicConfig{TR, TC, BLK}, K, SL, atomic_fn<K, TR, TC, BLK, SL>()},
 const AtomicVariant kAtomicVariants[] = {PARENT_ATOMIC_VARIANTS(PARENT_ATOMIC_ENTRY)};
 #undef PARENT_ATOMIC_ENTRY
 
@@ -315,10 +518,10 @@ int atomic_config_count() { return (int)(sizeof(kAtomicVariants) / sizeof(kAtomi
 AtomicConfig atomic_config(int id) { return kAtomicVariants[id].cfg; }
 size_t atomic_lds_bytes(int id) {
     const AtomicVariant& v = kAtomicVariants[id];
-    return (size_t)v.cfg.tr * v.cfg.tc * v.slotw * sizeof(unsigned);
+    return (size_t)v.cfg.tr * v.cfg.tc * v.slot * (v.kind == 1 ? 64 : 4);
 }
 bool atomic_fits(int id, int bmax, size_t lds_max) {
-    return (long long)bmax * bmax <= kAtomicVariants[id].slotw && atomic_lds_bytes(id) <= lds_max;
+    return (long long)bmax * bmax <= kAtomicVariants[id].slot && atomic_lds_bytes(id) <= lds_max;
 }
 
 hipError_t launch_pairs_atomic(const PairInputs& in, int id, const int* tiles, int ntiles,
@@ -334,10 +537,10 @@ hipError_t launch_pairs_atomic(const PairInputs& in, int id, const int* tiles, i
         const int chunk = (t1 - t0 + 7) / 8;
         dim3 grid((unsigned)chunk * 8);
         switch (id) {
-#define PARENT_ATOMIC_CASE(ID, TR, TC, BLK, SW)                                                  \
-    case ID:                                                                                      \
-        k_pairs_atomic<TR, TC, BLK, SW><<<grid, BLK, lds, s>>>(in, tiles, t0, t1, dump_begin,    \
-                                                               dump_entries, dump_out);           \
+#define PARENT_ATOMIC_CASE(ID, K, TR, TC, BLK, SL)                                              \
+    case ID:                                                                                     \
+        launch_variant<K, TR, TC, BLK, SL>(grid, lds, s, in, tiles, t0, t1, dump_begin,         \
+                                           dump_entries, dump_out);                              \
         break;
             PARENT_ATOMIC_VARIANTS(PARENT_ATOMIC_CASE)
 #undef PARENT_ATOMIC_CASE

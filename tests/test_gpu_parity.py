@@ -137,7 +137,8 @@ def test_every_atomic_variant_bit_exact(engine):
     sample, S2D and MI within tolerance on all pairs (fast and general fills)."""
     pa = _pa()
     for bins, atoms, frames in (([50] * 6, 26, 4097), ([10, 10, 10, 20, 30, 40], 22, 3000),
-                                ([75] * 6, 14, 2500), ([150] * 6, 10, 1500)):
+                                ([75] * 6, 14, 2500), ([150] * 6, 10, 1500), ([10] * 6, 20, 2000),
+                                ([30] * 6, 18, 2000), ([20] * 6, 16, 17000)):
         vals, nD = _case(atoms, frames, 13)
         engine.set_trajectory(vals)
         engine.prepare(bins)
