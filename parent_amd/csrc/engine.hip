@@ -9,6 +9,7 @@
 #include "kernels.h"
 
 #include <algorithm>
+#include <map>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -154,29 +155,72 @@ std::vector<int> make_tiles(int D, int tr, int tc) {
     return t;
 }
 
+// Work items of the row-stationary kernel: (row i, columns [j0, j1)) for
+// column segments of 32, segment-major then row, so workgroups started
+// together stream the same columns.
+constexpr int kItemSeg = 32;
+void push_items(std::vector<int>& t, int D, int r0, int r1) {
+    for (int j0 = ((r0 + 1) / kItemSeg) * kItemSeg; j0 < D; j0 += kItemSeg)
+        for (int i = r0; i < r1; i++) {
+            const int a = std::max(j0, i + 1), b = std::min(j0 + kItemSeg, D);
+            if (a < b) {
+                t.push_back(i);
+                t.push_back(a);
+                t.push_back(b);
+            }
+        }
+}
+
+std::vector<int> make_work(int D, int config) {
+    const AtomicConfig c = atomic_config(config);
+    if (atomic_kind(config) == 3) {
+        std::vector<int> t;
+        push_items(t, D, 0, D - 1);
+        return t;
+    }
+    return make_tiles(D, c.tr, c.tc);
+}
+
+int work_stride(int config) { return atomic_kind(config) == 3 ? 3 : 2; }
+
 int ensure_tiles(parent_engine* e, int config) {
     if (e->tiles.config == config && e->tiles.D == e->D) return PARENT_OK;
-    const AtomicConfig c = atomic_config(config);
-    std::vector<int> t = make_tiles(e->D, c.tr, c.tc);
-    HIPCHK(e, e->tiles.dev.ensure(std::max<size_t>(t.size(), 2) * sizeof(int)));
+    std::vector<int> t = make_work(e->D, config);
+    HIPCHK(e, e->tiles.dev.ensure(std::max<size_t>(t.size(), 3) * sizeof(int)));
     if (!t.empty())
         HIPCHK(e, hipMemcpyAsync(e->tiles.dev.p, t.data(), t.size() * sizeof(int), hipMemcpyHostToDevice, e->stream));
     HIPCHK(e, hipStreamSynchronize(e->stream));
     e->tiles.config = config;
     e->tiles.D = e->D;
-    e->tiles.ntiles = (int)(t.size() / 2);
+    e->tiles.ntiles = (int)(t.size() / work_stride(config));
     return PARENT_OK;
 }
 
 // Largest tile configuration whose histograms fit the LDS of one workgroup.
+bool variant_fits(const parent_engine* e, int id) {
+    if (!atomic_fits(id, bmax2d(e), (size_t)e->lds_max)) return false;
+    return atomic_kind(id) != 3 || e->N <= kRowstatMaxFrames;
+}
+
+// one launch of atomic-method configuration `id` over a work list
+hipError_t launch_variant(parent_engine* e, const PairInputs& in, int id, const int* work, int nwork,
+                          const int* db, const int* de, int* dout) {
+    if (atomic_kind(id) == 3) {
+        const int niter = (e->N + 16383) / 16384;
+        return launch_pairs_rowstat(in, niter, work, nwork, db, de, dout, e->stream);
+    }
+    return launch_pairs_atomic(in, id, work, nwork, db, de, dout, e->stream);
+}
+
 int choose_atomic_config(const parent_engine* e) {
     const int bmax = bmax2d(e);
-    if (e->forced_atomic >= 0) return atomic_fits(e->forced_atomic, bmax, (size_t)e->lds_max) ? e->forced_atomic : -1;
+    if (e->forced_atomic >= 0) return variant_fits(e, e->forced_atomic) ? e->forced_atomic : -1;
     // fallback order when not autotuning: the best of the autotuner's tables
     // at 50 bins first, then configurations with larger histogram slots
     static const int order[] = {2, 3, 1, 9, 0, 11, 13, 10, 15, 16, 14, 18};
     for (int id : order)
-        if (atomic_fits(id, bmax, (size_t)e->lds_max)) return id;
+        if (variant_fits(e, id)) return id;
+    (void)bmax;
     return -1;
 }
 
@@ -198,9 +242,13 @@ std::vector<std::pair<int, int>> tune_rows(int D) {
     return rows;
 }
 
-std::vector<int> tune_tiles(int D, int tr, int tc, const std::vector<std::pair<int, int>>& rows) {
+std::vector<int> tune_work(int D, int config, const std::vector<std::pair<int, int>>& rows) {
     std::vector<int> t;
-    for (auto& rr : rows) push_band(t, D, tr, tc, rr.first, rr.second);
+    const AtomicConfig c = atomic_config(config);
+    for (auto& rr : rows) {
+        if (atomic_kind(config) == 3) push_items(t, D, rr.first, rr.second);
+        else push_band(t, D, c.tr, c.tc, rr.first, rr.second);
+    }
     return t;
 }
 
@@ -231,13 +279,13 @@ int run_autotune(parent_engine* e) {
     double best_rate = -1;
     for (int id = 0; id < atomic_config_count(); id++) {
         const AtomicConfig c = atomic_config(id);
-        if (!atomic_fits(id, bmax, (size_t)e->lds_max)) continue;
-        std::vector<int> t = tune_tiles(D, c.tr, c.tc, rows);
+        if (!variant_fits(e, id)) continue;
+        std::vector<int> t = tune_work(D, id, rows);
         HIPCHK(e, dt.ensure(t.size() * sizeof(int)));
         HIPCHK(e, hipMemcpyAsync(dt.p, t.data(), t.size() * sizeof(int), hipMemcpyHostToDevice, e->stream));
         double ms = 0;
-        const int nt = (int)(t.size() / 2);
-        if (int rc = time_launch(e, [&] { return launch_pairs_atomic(in, id, dt.as<int>(), nt, nullptr, nullptr, nullptr, e->stream); }, &ms))
+        const int nt = (int)(t.size() / work_stride(id));
+        if (int rc = time_launch(e, [&] { return launch_variant(e, in, id, dt.as<int>(), nt, nullptr, nullptr, nullptr); }, &ms))
             return rc;
         parent_tune_entry te{PARENT_METHOD_ATOMIC, id, c.tr, c.tc, c.block, ms, updates, updates / (ms * 1e-3)};
         e->last_tune.push_back(te);
@@ -436,7 +484,7 @@ int parent_engine_pairs(parent_engine* e, int method) {
     if (used == PARENT_METHOD_PLAIN) {
         HIPCHK(e, launch_pairs_plain(in, nullptr, e->stream));
     } else if (used == PARENT_METHOD_ATOMIC) {
-        HIPCHK(e, launch_pairs_atomic(in, variant, e->tiles.dev.as<int>(), e->tiles.ntiles, nullptr, nullptr, nullptr, e->stream));
+        HIPCHK(e, launch_variant(e, in, variant, e->tiles.dev.as<int>(), e->tiles.ntiles, nullptr, nullptr, nullptr));
     } else {
         return fail(e, PARENT_E_INVALID, "unknown method");
     }
@@ -518,24 +566,29 @@ int parent_engine_pair_counts(parent_engine* e, int method, int n_pairs_req, con
         CountsDump dump{n_pairs_req, dpairs.as<int>(), doff.as<long long>(), dout.as<int>()};
         HIPCHK(e, launch_pairs_plain(in, &dump, e->stream));
     } else if (used == PARENT_METHOD_ATOMIC) {
-        // the tiles holding the requested pairs, each pair as (slot, offset)
+        // the tiles (or, for the row-stationary kind, one-column work items)
+        // holding the requested pairs, each pair as (slot, offset)
         const AtomicConfig c = atomic_config(variant);
+        const bool items = atomic_kind(variant) == 3;
+        const int stride = items ? 3 : 2;
         std::vector<std::vector<int>> per_tile;
         std::vector<int> tl;
-        std::vector<std::pair<long long, int>> key;  // tile key -> index
+        std::map<long long, int> key;  // tile key -> index
         for (int k = 0; k < n_pairs_req; k++) {
             const int gi = pairs[2 * k], gj = pairs[2 * k + 1];
-            const int r0 = gi / c.tr * c.tr, c0 = gj / c.tc * c.tc;
+            const int r0 = items ? gi : gi / c.tr * c.tr, c0 = items ? gj : gj / c.tc * c.tc;
             const long long tk = (long long)r0 * e->D + c0;
-            int idx = -1;
-            for (auto& kv : key)
-                if (kv.first == tk) idx = kv.second;
-            if (idx < 0) {
-                idx = (int)tl.size() / 2;
-                key.push_back({tk, idx});
+            auto it = key.find(tk);
+            int idx;
+            if (it == key.end()) {
+                idx = (int)tl.size() / stride;
+                key[tk] = idx;
                 tl.push_back(r0);
                 tl.push_back(c0);
+                if (items) tl.push_back(c0 + 1);
                 per_tile.emplace_back();
+            } else {
+                idx = it->second;
             }
             per_tile[idx].push_back(k);
         }
@@ -543,8 +596,8 @@ int parent_engine_pair_counts(parent_engine* e, int method, int n_pairs_req, con
         for (size_t t = 0; t < per_tile.size(); t++) {
             for (int k : per_tile[t]) {
                 const int gi = pairs[2 * k], gj = pairs[2 * k + 1];
-                const int r0 = tl[2 * t], c0 = tl[2 * t + 1];
-                entries.push_back((gi - r0) * c.tc + (gj - c0));
+                const int r0 = tl[stride * t], c0 = tl[stride * t + 1];
+                entries.push_back(items ? 0 : (gi - r0) * c.tc + (gj - c0));
                 entries.push_back((int)(off[k] >> 31));
                 entries.push_back((int)(off[k] & 0x7FFFFFFF));
             }
@@ -557,7 +610,7 @@ int parent_engine_pair_counts(parent_engine* e, int method, int n_pairs_req, con
         HIPCHK(e, hipMemcpyAsync(dtiles.p, tl.data(), tl.size() * 4, hipMemcpyHostToDevice, e->stream));
         HIPCHK(e, hipMemcpyAsync(dbegin.p, begin.data(), begin.size() * 4, hipMemcpyHostToDevice, e->stream));
         HIPCHK(e, hipMemcpyAsync(dent.p, entries.data(), entries.size() * 4, hipMemcpyHostToDevice, e->stream));
-        HIPCHK(e, launch_pairs_atomic(in, variant, dtiles.as<int>(), (int)(tl.size() / 2), dbegin.as<int>(), dent.as<int>(), dout.as<int>(), e->stream));
+        HIPCHK(e, launch_variant(e, in, variant, dtiles.as<int>(), (int)(tl.size() / stride), dbegin.as<int>(), dent.as<int>(), dout.as<int>()));
         HIPCHK(e, hipStreamSynchronize(e->stream));
         dtiles.release();
         dbegin.release();

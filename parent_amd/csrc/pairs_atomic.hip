@@ -567,6 +567,153 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_sliced_persistent(PairInputs in
     }
 }
 
+// Row-stationary sliced kernel (1 x 1 pairs, one 1024-thread workgroup per CU).
+// Work items (row dof i, columns [j0, j1)) are ordered column-segment major,
+// so the workgroups running together stream the same column-dof rows through
+// L2.  The row's bin indices stay in registers for the whole item (NITER
+// uint4 per thread, frames tid*16 + it*16384), so a pair costs one column
+// stream (1 byte per update).  Sub-counters are never cleared between pairs:
+// each thread keeps the previous sums of the cell quarters it reads in the
+// epilogue and takes the difference (mod 2^32, exact for any per-pair count
+// < 2^32); they are cleared only when the histogram shape (bins_i, bins_j)
+// changes.  The epilogue reads a cell's 64 bytes with 4 lanes (conflict-free
+// ds_read_b128) and sums the quarters with lane shuffles.
+template <int NITER, bool PREFETCH>
+__global__ __launch_bounds__(1024) void k_pairs_rowstat(PairInputs in, const int* __restrict__ items, int nitems,
+                                                        const int* __restrict__ dump_begin,
+                                                        const int* __restrict__ dump_entries,
+                                                        int* __restrict__ dump_out) {
+    constexpr int kBlock = 1024, kSlotC = 2500, kSlotW = kSlotC * 16, kWaves = kBlock / 64;
+    constexpr int kPerIter = kBlock * 16;
+    constexpr int kCellsPerPass = kBlock / 4;
+    constexpr int kK = (kSlotC + kCellsPerPass - 1) / kCellsPerPass;
+    extern __shared__ unsigned hist[];
+    __shared__ double nls[256];
+    __shared__ double sT[kWaves];
+    __shared__ int sO[kWaves];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const unsigned s16 = tid & 15;
+    const int q = tid & 3, cbase = tid >> 2;
+    const int D = in.D, N = in.N, nD = in.nD;
+    const int niter = (N + kPerIter - 1) / kPerIter;  // <= NITER (host checks)
+    for (int n = tid; n < 256; n += kBlock) nls[n] = in.nlogn[n < N ? n : 0];
+    unsigned prev[kK];
+#pragma unroll
+    for (int k = 0; k < kK; k++) prev[k] = 0;
+    int curBi = -1, curBj = -1;
+
+    auto load_row = [&](const uint8_t* base, uint4* X) {
+#pragma unroll
+        for (int it = 0; it < NITER; it++) {
+            const int f = it * kPerIter + tid * 16;
+            X[it] = (it < niter && f < in.Np) ? *reinterpret_cast<const uint4*>(base + f) : make_uint4(0, 0, 0, 0);
+        }
+    };
+    for (int item = blockIdx.x; item < nitems; item += gridDim.x) {
+        const int i = items[3 * item], j0 = items[3 * item + 1], j1 = items[3 * item + 2];
+        const int Bi = in.B.of(dof_type(nD, i));
+        uint4 R[NITER], C[NITER];
+        load_row(in.bins + (size_t)i * in.Np, R);
+        load_row(in.bins + (size_t)j0 * in.Np, C);
+        for (int j = j0; j < j1; j++) {
+            const int Bj = in.B.of(dof_type(nD, j));
+            if (Bi != curBi || Bj != curBj) {
+                __syncthreads();
+                uint4* h4 = reinterpret_cast<uint4*>(hist);
+                for (int w = tid; w < kSlotW / 4; w += kBlock) h4[w] = make_uint4(0, 0, 0, 0);
+#pragma unroll
+                for (int k = 0; k < kK; k++) prev[k] = 0;
+                curBi = Bi;
+                curBj = Bj;
+                __syncthreads();
+            }
+            const unsigned B16 = (unsigned)Bj * 16u;
+#pragma unroll
+            for (int it = 0; it < NITER; it++) {
+                if (it >= niter) break;
+                const int f = it * kPerIter + tid * 16;
+                if (f + 16 <= N) {
+#pragma unroll
+                    for (int e = 0; e < 16; e++)
+                        atomicAdd(&hist[__umul24(byte_of(R[it], e), B16) + (byte_of(C[it], e) << 4) + s16], 1u);
+                } else {
+                    const int nvalid = max(0, N - f);
+                    for (int e = 0; e < nvalid; e++)
+                        atomicAdd(&hist[__umul24(byte_of(R[it], e), B16) + (byte_of(C[it], e) << 4) + s16], 1u);
+                }
+            }
+            if (PREFETCH && j + 1 < j1) load_row(in.bins + (size_t)(j + 1) * in.Np, C);
+            __syncthreads();
+            // epilogue: 4 lanes per cell, each reads one 16-byte quarter
+            const int cells = Bi * Bj;
+            double T = 0.0;
+            int occ = 0;
+            long long doff = -1;
+            if (dump_out) {
+                for (int e = dump_begin[item]; e < dump_begin[item + 1]; e++)
+                    if (dump_entries[3 * e] == j - j0)
+                        doff = ((long long)dump_entries[3 * e + 1] << 31) | (long long)dump_entries[3 * e + 2];
+            }
+#pragma unroll
+            for (int k = 0; k < kK; k++) {
+                const int c = cbase + k * kCellsPerPass;
+                unsigned qs = 0;
+                if (c < cells) {
+                    const uint4 v = *reinterpret_cast<const uint4*>(hist + c * 16 + q * 4);
+                    qs = v.x + v.y + v.z + v.w;
+                }
+                unsigned d = qs - prev[k];
+                prev[k] = qs;
+                d += __shfl_xor(d, 1, 64);
+                d += __shfl_xor(d, 2, 64);
+                if (q == 0 && c < cells) {
+                    if (doff >= 0) dump_out[doff + c] = (int)d;
+                    if (d) {
+                        T += d < 256u ? nls[d] : (double)d * log((double)d);
+                        occ++;
+                    }
+                }
+            }
+            T = wave_sum(T);
+            occ = wave_sum(occ);
+            if (lane == 0) {
+                sT[wid] = T;
+                sO[wid] = occ;
+            }
+            __syncthreads();
+            if (tid == 0) {
+                double tt = 0.0;
+                int oo = 0;
+                for (int w = 0; w < kWaves; w++) {
+                    tt += sT[w];
+                    oo += sO[w];
+                }
+                in.s2d[par_offset(nD, i, j)] = ((in.logN + in.c2[i]) + in.c2[j] + (oo - 1.0) / (2.0 * N)) - tt / N;
+            }
+            if (!PREFETCH && j + 1 < j1) load_row(in.bins + (size_t)(j + 1) * in.Np, C);
+        }
+    }
+}
+
+hipError_t launch_pairs_rowstat(const PairInputs& in, int niter_max, const int* items, int nitems,
+                                const int* dump_begin, const int* dump_entries, int* dump_out,
+                                hipStream_t s) {
+    const size_t lds = (size_t)2500 * 64;
+    int dev = 0, cus = 0;
+    hipError_t e;
+    if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+    if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
+    const int grid = std::max(1, std::min(cus, nitems));
+    const void* fn = niter_max <= 7 ? reinterpret_cast<const void*>(&k_pairs_rowstat<7, true>)
+                                    : reinterpret_cast<const void*>(&k_pairs_rowstat<14, false>);
+    if ((e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)) != hipSuccess) return e;
+    if (niter_max <= 7)
+        k_pairs_rowstat<7, true><<<grid, 1024, lds, s>>>(in, items, nitems, dump_begin, dump_entries, dump_out);
+    else
+        k_pairs_rowstat<14, false><<<grid, 1024, lds, s>>>(in, items, nitems, dump_begin, dump_entries, dump_out);
+    return hipGetLastError();
+}
+
 namespace {
 
 // (kind, tile rows, tile columns, threads, slot size).  kind 0: plain LDS
@@ -574,7 +721,8 @@ namespace {
 // 10240 -> B <= 101, 40960 -> B <= 202); kind 1: sliced histograms, slot size
 // in cells (64 bytes each; 2500 -> B <= 50, 1225 -> B <= 35, 900 -> B <= 30,
 // 400 -> B <= 20, 100 -> B <= 10); kind 2: persistent sliced, 1 x 1 tiles,
-// slot size in cells.
+// slot size in cells; kind 3: row-stationary sliced (k_pairs_rowstat), run
+// on work items instead of tiles.
 #define PARENT_ATOMIC_VARIANTS(X)     \
     X(0, 0, 1, 1, 256, 2560)          \
     X(1, 0, 2, 2, 256, 2560)          \
@@ -613,7 +761,8 @@ namespace {
     X(34, 2, 1, 1, 512, 900)          \
     X(35, 2, 1, 1, 512, 400)          \
     X(36, 2, 1, 1, 256, 400)          \
-    X(37, 2, 1, 1, 256, 100)
+    X(37, 2, 1, 1, 256, 100)          \
+    X(38, 3, 1, 1, 1024, 2500)
 
 struct AtomicVariant {
     AtomicConfig cfg;
@@ -626,7 +775,8 @@ template <int KIND, int TR, int TC, int BLK, int SL>
 const void* atomic_fn() {
     if constexpr (KIND == 0) return reinterpret_cast<const void*>(&k_pairs_atomic<TR, TC, BLK, SL>);
     else if constexpr (KIND == 1) return reinterpret_cast<const void*>(&k_pairs_sliced<TR, TC, BLK, SL>);
-    else return reinterpret_cast<const void*>(&k_pairs_sliced_persistent<BLK, SL>);
+    else if constexpr (KIND == 2) return reinterpret_cast<const void*>(&k_pairs_sliced_persistent<BLK, SL>);
+    else return reinterpret_cast<const void*>(&k_pairs_rowstat<7, true>);
 }
 
 template <int KIND, int TR, int TC, int BLK, int SL>
@@ -647,6 +797,7 @@ const AtomicVariant kAtomicVariants[] = {PARENT_ATOMIC_VARIANTS(PARENT_ATOMIC_EN
 
 int atomic_config_count() { return (int)(sizeof(kAtomicVariants) / sizeof(kAtomicVariants[0])); }
 AtomicConfig atomic_config(int id) { return kAtomicVariants[id].cfg; }
+int atomic_kind(int id) { return kAtomicVariants[id].kind; }
 size_t atomic_lds_bytes(int id) {
     const AtomicVariant& v = kAtomicVariants[id];
     return (size_t)v.cfg.tr * v.cfg.tc * v.slot * (v.kind == 0 ? 4 : 64);
