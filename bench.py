@@ -81,6 +81,7 @@ def main():
     ap.add_argument("--cpu-sample-atoms", type=int, default=500)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tune-report", action="store_true", help="print the autotuner's table")
+    ap.add_argument("--variant", type=int, default=-1, help="pin the atomic method's configuration")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (--gpus {args.gpus} requested); "
@@ -104,6 +105,8 @@ def main():
     eng.set_trajectory(vals)
     del vals
     b6 = [bins] * 6
+    if args.variant >= 0:
+        eng.set_variant(pa.METHOD_ATOMIC, args.variant)
 
     def step(m):
         eng.prepare(b6)

@@ -62,6 +62,48 @@ __global__ __launch_bounds__(kBlock) void k_lds(unsigned* out, int iters) {
     out[(size_t)blockIdx.x * kBlock + threadIdx.x] = s;
 }
 
+// The sliced-histogram pattern over a dynamic table of `words` words, one
+// 1024-thread workgroup per CU: word = cell * 16 + lane % 16 with random
+// cells in [0, words / 16) (MODE 0), or uniform random words (MODE 1).
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_lds_big(unsigned* out, int iters, int words) {
+    extern __shared__ unsigned hb[];
+    for (int i = threadIdx.x; i < words; i += kBlock) hb[i] = 0;
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    unsigned addr[kAddr];
+#pragma unroll
+    for (int k = 0; k < kAddr; k++) {
+        const unsigned h = hash32(blockIdx.x * 7919u + threadIdx.x * 131u + k);
+        addr[k] = MODE == 0 ? (h % (unsigned)(words / 16)) * 16u + (lane & 15) : h % (unsigned)words;
+    }
+    for (int it = 0; it < iters; it++) {
+#pragma unroll
+        for (int k = 0; k < kAddr; k++) atomicAdd(&hb[addr[k]], 1u);
+    }
+    __syncthreads();
+    unsigned s = 0;
+    for (int i = threadIdx.x; i < words; i += kBlock) s += hb[i];
+    out[(size_t)blockIdx.x * kBlock + threadIdx.x] = s;
+}
+
+template <int MODE>
+double run_big(int blocks, int iters, int words, unsigned* dout) {
+    const size_t lds = (size_t)words * 4;
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_lds_big<MODE>), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipEvent_t a, b;
+    (void)hipEventCreate(&a);
+    (void)hipEventCreate(&b);
+    k_lds_big<MODE><<<blocks, kBlock, lds>>>(dout, 4, words);
+    (void)hipEventRecord(a);
+    k_lds_big<MODE><<<blocks, kBlock, lds>>>(dout, iters, words);
+    (void)hipEventRecord(b);
+    (void)hipEventSynchronize(b);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, a, b);
+    return (double)blocks * kBlock * kAddr * iters / (ms * 1e-3);
+}
+
 template <int MODE>
 double run(int blocks, int iters, unsigned* dout) {
     hipEvent_t a, b;
@@ -99,12 +141,17 @@ int main() {
     const double r5 = run<5>(blocks, 256, dout);
     const double r6 = run<6>(blocks, 256, dout);
     const double r7 = run<7>(blocks, 2048, dout);
+    const double r8 = run_big<0>(cus, 2048, 40000, dout);  // sliced, 160 KB, 1 WG/CU
+    const double r9 = run_big<0>(cus, 2048, 10000, dout);  // sliced, 40 KB, 1 WG/CU
+    const double r10 = run_big<1>(cus, 2048, 40000, dout); // random, 160 KB, 1 WG/CU
+    const double r11 = run_big<0>(cus * 2, 2048, 20000, dout);  // sliced, 80 KB, 2 WG/CU
     auto per = [&](double r) { return r / (cus * clk); };
     printf("{\"device\": \"%s\", \"cus\": %d, \"clock_mhz\": %.0f", p.gcnArchName, cus, clk / 1e6);
     const char* names[] = {"conflict_free", "random_2500", "same_address", "stride_2", "stride_16",
-                           "stride_32", "stride_64", "lane_sliced"};
-    const double rates[] = {r0, r1, r2, r3, r4, r5, r6, r7};
-    for (int k = 0; k < 8; k++)
+                           "stride_32", "stride_64", "lane_sliced", "sliced_160KB_1wg", "sliced_40KB_1wg",
+                           "random_160KB_1wg", "sliced_80KB_2wg"};
+    const double rates[] = {r0, r1, r2, r3, r4, r5, r6, r7, r8, r9, r10, r11};
+    for (int k = 0; k < 12; k++)
         printf(", \"%s\": {\"atomics_per_s\": %.4e, \"per_clk_per_cu\": %.2f}", names[k], rates[k], per(rates[k]));
     printf("}\n");
     (void)hipFree(dout);
