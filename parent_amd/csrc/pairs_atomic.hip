@@ -288,7 +288,7 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_atomic(PairInputs in, const int
 // sub-counters (epilogue).  64 bytes per cell: one 50 x 50 histogram fills
 // the 160 KB of LDS.
 template <int TR, int TC, int BLOCK, int SLOTC>
-__global__ __launch_bounds__(BLOCK) void k_pairs_sliced(PairInputs in, const int* __restrict__ tiles,
+__global__ __launch_bounds__(BLOCK, 1) void k_pairs_sliced(PairInputs in, const int* __restrict__ tiles,
                                                         int t_begin, int t_end,
                                                         const int* __restrict__ dump_begin,
                                                         const int* __restrict__ dump_entries,
@@ -454,7 +454,7 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_sliced(PairInputs in, const int
 // next pair's first words are loaded before the epilogue, the epilogue clears
 // the sub-counters it reads, and n ln n is computed in registers.
 template <int BLOCK, int SLOTC>
-__global__ __launch_bounds__(BLOCK) void k_pairs_sliced_persistent(PairInputs in, const int* __restrict__ tiles,
+__global__ __launch_bounds__(BLOCK, 1) void k_pairs_sliced_persistent(PairInputs in, const int* __restrict__ tiles,
                                                                    int ntiles,
                                                                    const int* __restrict__ dump_begin,
                                                                    const int* __restrict__ dump_entries,
@@ -579,7 +579,7 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_sliced_persistent(PairInputs in
 // changes.  The epilogue reads a cell's 64 bytes with 4 lanes (conflict-free
 // ds_read_b128) and sums the quarters with lane shuffles.
 template <int NITER, bool PREFETCH>
-__global__ __launch_bounds__(1024) void k_pairs_rowstat(PairInputs in, const int* __restrict__ items, int nitems,
+__global__ __launch_bounds__(1024, 1) void k_pairs_rowstat(PairInputs in, const int* __restrict__ items, int nitems,
                                                         const int* __restrict__ dump_begin,
                                                         const int* __restrict__ dump_entries,
                                                         int* __restrict__ dump_out) {
@@ -627,7 +627,9 @@ __global__ __launch_bounds__(1024) void k_pairs_rowstat(PairInputs in, const int
                 curBj = Bj;
                 __syncthreads();
             }
-            const unsigned B16 = (unsigned)Bj * 16u;
+            // byte address of sub-counter (cell a*Bj + b, slice s) = a*64*Bj + b*64 + 4*s
+            const unsigned B64 = (unsigned)Bj * 64u, s4 = s16 * 4u;
+            char* hb = reinterpret_cast<char*>(hist);
 #pragma unroll
             for (int it = 0; it < NITER; it++) {
                 if (it >= niter) break;
@@ -635,11 +637,11 @@ __global__ __launch_bounds__(1024) void k_pairs_rowstat(PairInputs in, const int
                 if (f + 16 <= N) {
 #pragma unroll
                     for (int e = 0; e < 16; e++)
-                        atomicAdd(&hist[__umul24(byte_of(R[it], e), B16) + (byte_of(C[it], e) << 4) + s16], 1u);
+                        atomicAdd(reinterpret_cast<unsigned*>(hb + (__umul24(byte_of(R[it], e), B64) + (byte_of(C[it], e) << 6) + s4)), 1u);
                 } else {
                     const int nvalid = max(0, N - f);
                     for (int e = 0; e < nvalid; e++)
-                        atomicAdd(&hist[__umul24(byte_of(R[it], e), B16) + (byte_of(C[it], e) << 4) + s16], 1u);
+                        atomicAdd(reinterpret_cast<unsigned*>(hb + (__umul24(byte_of(R[it], e), B64) + (byte_of(C[it], e) << 6) + s4)), 1u);
                 }
             }
             if (PREFETCH && j + 1 < j1) load_row(in.bins + (size_t)(j + 1) * in.Np, C);
