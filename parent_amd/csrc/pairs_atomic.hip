@@ -338,15 +338,28 @@ __global__ __launch_bounds__(BLOCK, 1) void k_pairs_sliced(PairInputs in, const 
     for (int c = 0; c < TC; c++) cols[c] = in.bins + (size_t)(gc[c] < D ? gc[c] : 0) * in.Np;
 
     int it0 = 0;
-    if (fast) {
+    if (fast && nfull > 0) {
         const unsigned B16 = (unsigned)Bc[0] * 16u;
+        // software pipeline: the next iteration's index words are loaded
+        // before this iteration's atomics are issued
+        uint4 Rn[TR], Cn[TC];
+#pragma unroll
+        for (int r = 0; r < TR; r++) Rn[r] = *reinterpret_cast<const uint4*>(rows[r] + threadIdx.x * 16);
+#pragma unroll
+        for (int c = 0; c < TC; c++) Cn[c] = *reinterpret_cast<const uint4*>(cols[c] + threadIdx.x * 16);
         for (int it = 0; it < nfull; it++) {
-            const int f = it * per_iter + threadIdx.x * 16;
             uint4 R[TR], C[TC];
 #pragma unroll
-            for (int r = 0; r < TR; r++) R[r] = *reinterpret_cast<const uint4*>(rows[r] + f);
+            for (int r = 0; r < TR; r++) R[r] = Rn[r];
 #pragma unroll
-            for (int c = 0; c < TC; c++) C[c] = *reinterpret_cast<const uint4*>(cols[c] + f);
+            for (int c = 0; c < TC; c++) C[c] = Cn[c];
+            const int fn = (it + 1) * per_iter + threadIdx.x * 16;
+            if (fn < in.Np) {
+#pragma unroll
+                for (int r = 0; r < TR; r++) Rn[r] = *reinterpret_cast<const uint4*>(rows[r] + fn);
+#pragma unroll
+                for (int c = 0; c < TC; c++) Cn[c] = *reinterpret_cast<const uint4*>(cols[c] + fn);
+            }
 #pragma unroll
             for (int e = 0; e < 16; e++) {
                 unsigned ro[TR], co[TC];
@@ -464,9 +477,11 @@ __global__ __launch_bounds__(BLOCK, 1) void k_pairs_sliced_persistent(PairInputs
     constexpr int kWaves = BLOCK / 64;
     __shared__ double sT[kWaves];
     __shared__ int sOcc[kWaves];
+    __shared__ double nls[256];
     const int D = in.D, N = in.N, nD = in.nD;
     const unsigned s16 = threadIdx.x & 15;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int n = threadIdx.x; n < 256; n += BLOCK) nls[n] = in.nlogn[n <= N ? n : 0];
     // XCD x = blockIdx % 8 owns tiles [x*per, (x+1)*per); its workgroups stride through them
     const int per = (ntiles + 7) / 8;
     const int x = blockIdx.x % 8, wx = blockIdx.x / 8, gx = gridDim.x / 8;
@@ -542,7 +557,7 @@ __global__ __launch_bounds__(BLOCK, 1) void k_pairs_sliced_persistent(PairInputs
                 q[k] = make_uint4(0, 0, 0, 0);
             }
             if (n) {
-                T += (double)n * log((double)n);
+                T += n < 256u ? nls[n] : in.nlogn[n];
                 occ++;
             }
         }
