@@ -24,7 +24,7 @@ struct DevBuf {
     size_t n = 0;
     hipError_t ensure(size_t bytes) {
         if (bytes <= n && p) return hipSuccess;
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
         p = nullptr;
         n = 0;
         if (bytes == 0) return hipSuccess;
@@ -33,7 +33,7 @@ struct DevBuf {
         return e;
     }
     void release() {
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
         p = nullptr;
         n = 0;
     }
@@ -155,60 +155,26 @@ std::vector<int> make_tiles(int D, int tr, int tc) {
     return t;
 }
 
-// Work items of the row-stationary kernel: (row i, columns [j0, j1)) for
-// column segments of 32, segment-major then row, so workgroups started
-// together stream the same columns.
-constexpr int kItemSeg = 32;
-void push_items(std::vector<int>& t, int D, int r0, int r1) {
-    for (int j0 = ((r0 + 1) / kItemSeg) * kItemSeg; j0 < D; j0 += kItemSeg)
-        for (int i = r0; i < r1; i++) {
-            const int a = std::max(j0, i + 1), b = std::min(j0 + kItemSeg, D);
-            if (a < b) {
-                t.push_back(i);
-                t.push_back(a);
-                t.push_back(b);
-            }
-        }
-}
-
-std::vector<int> make_work(int D, int config) {
-    const AtomicConfig c = atomic_config(config);
-    if (atomic_kind(config) == 3) {
-        std::vector<int> t;
-        push_items(t, D, 0, D - 1);
-        return t;
-    }
-    return make_tiles(D, c.tr, c.tc);
-}
-
-int work_stride(int config) { return atomic_kind(config) == 3 ? 3 : 2; }
-
 int ensure_tiles(parent_engine* e, int config) {
     if (e->tiles.config == config && e->tiles.D == e->D) return PARENT_OK;
-    std::vector<int> t = make_work(e->D, config);
+    const AtomicConfig c = atomic_config(config);
+    std::vector<int> t = make_tiles(e->D, c.tr, c.tc);
     HIPCHK(e, e->tiles.dev.ensure(std::max<size_t>(t.size(), 3) * sizeof(int)));
     if (!t.empty())
         HIPCHK(e, hipMemcpyAsync(e->tiles.dev.p, t.data(), t.size() * sizeof(int), hipMemcpyHostToDevice, e->stream));
     HIPCHK(e, hipStreamSynchronize(e->stream));
     e->tiles.config = config;
     e->tiles.D = e->D;
-    e->tiles.ntiles = (int)(t.size() / work_stride(config));
+    e->tiles.ntiles = (int)(t.size() / 2);
     return PARENT_OK;
 }
 
 // Largest tile configuration whose histograms fit the LDS of one workgroup.
-bool variant_fits(const parent_engine* e, int id) {
-    if (!atomic_fits(id, bmax2d(e), (size_t)e->lds_max)) return false;
-    return atomic_kind(id) != 3 || e->N <= kRowstatMaxFrames;
-}
+bool variant_fits(const parent_engine* e, int id) { return atomic_fits(id, bmax2d(e), (size_t)e->lds_max); }
 
 // one launch of atomic-method configuration `id` over a work list
 hipError_t launch_variant(parent_engine* e, const PairInputs& in, int id, const int* work, int nwork,
                           const int* db, const int* de, int* dout) {
-    if (atomic_kind(id) == 3) {
-        const int niter = (e->N + 16383) / 16384;
-        return launch_pairs_rowstat(in, niter, work, nwork, db, de, dout, e->stream);
-    }
     return launch_pairs_atomic(in, id, work, nwork, db, de, dout, e->stream);
 }
 
@@ -245,10 +211,7 @@ std::vector<std::pair<int, int>> tune_rows(int D) {
 std::vector<int> tune_work(int D, int config, const std::vector<std::pair<int, int>>& rows) {
     std::vector<int> t;
     const AtomicConfig c = atomic_config(config);
-    for (auto& rr : rows) {
-        if (atomic_kind(config) == 3) push_items(t, D, rr.first, rr.second);
-        else push_band(t, D, c.tr, c.tc, rr.first, rr.second);
-    }
+    for (auto& rr : rows) push_band(t, D, c.tr, c.tc, rr.first, rr.second);
     return t;
 }
 
@@ -267,7 +230,7 @@ int time_launch(parent_engine* e, F launch, double* ms) {
 
 int run_autotune(parent_engine* e) {
     e->last_tune.clear();
-    const int D = e->D, bmax = bmax2d(e);
+    const int D = e->D;
     const auto rows = tune_rows(D);
     long long npairs = 0;
     for (auto& rr : rows)
@@ -284,7 +247,7 @@ int run_autotune(parent_engine* e) {
         HIPCHK(e, dt.ensure(t.size() * sizeof(int)));
         HIPCHK(e, hipMemcpyAsync(dt.p, t.data(), t.size() * sizeof(int), hipMemcpyHostToDevice, e->stream));
         double ms = 0;
-        const int nt = (int)(t.size() / work_stride(id));
+        const int nt = (int)(t.size() / 2);
         if (int rc = time_launch(e, [&] { return launch_variant(e, in, id, dt.as<int>(), nt, nullptr, nullptr, nullptr); }, &ms))
             return rc;
         parent_tune_entry te{PARENT_METHOD_ATOMIC, id, c.tr, c.tc, c.block, ms, updates, updates / (ms * 1e-3)};
@@ -339,7 +302,13 @@ int parent_engine_create(int device, parent_engine** out) {
         delete e;
         return PARENT_E_DEVICE;
     }
-    for (auto& ev : e->ev) hipEventCreate(&ev);
+    for (auto& ev : e->ev) {
+        if ((h = hipEventCreate(&ev)) != hipSuccess) {
+            set_last_error(std::string("hipEventCreate failed: ") + hipGetErrorString(h));
+            parent_engine_destroy(e);
+            return PARENT_E_DEVICE;
+        }
+    }
     int lds = 0;
     if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
         e->lds_max = lds;
@@ -349,16 +318,16 @@ int parent_engine_create(int device, parent_engine** out) {
 
 void parent_engine_destroy(parent_engine* e) {
     if (!e) return;
-    hipSetDevice(e->device);
-    if (e->stream) hipStreamSynchronize(e->stream);
+    (void)hipSetDevice(e->device);
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
     DevBuf* bufs[] = {&e->values_own, &e->emin, &e->emax, &e->mfhist, &e->flags, &e->vmin, &e->vmax,
                       &e->modfit, &e->bins2, &e->bins1, &e->s1d, &e->c2, &e->jtab, &e->nlogn, &e->s2d,
                       &e->mkey, &e->mupar, &e->minpos, &e->mstate, &e->medges, &e->mmi, &e->mleft,
                       &e->tiles.dev};
     for (DevBuf* b : bufs) b->release();
     for (auto& ev : e->ev)
-        if (ev) hipEventDestroy(ev);
-    if (e->stream) hipStreamDestroy(e->stream);
+        if (ev) (void)hipEventDestroy(ev);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
 }
 
@@ -566,17 +535,15 @@ int parent_engine_pair_counts(parent_engine* e, int method, int n_pairs_req, con
         CountsDump dump{n_pairs_req, dpairs.as<int>(), doff.as<long long>(), dout.as<int>()};
         HIPCHK(e, launch_pairs_plain(in, &dump, e->stream));
     } else if (used == PARENT_METHOD_ATOMIC) {
-        // the tiles (or, for the row-stationary kind, one-column work items)
-        // holding the requested pairs, each pair as (slot, offset)
+        // the tiles holding the requested pairs, each pair as (slot, offset)
         const AtomicConfig c = atomic_config(variant);
-        const bool items = atomic_kind(variant) == 3;
-        const int stride = items ? 3 : 2;
+        const int stride = 2;
         std::vector<std::vector<int>> per_tile;
         std::vector<int> tl;
         std::map<long long, int> key;  // tile key -> index
         for (int k = 0; k < n_pairs_req; k++) {
             const int gi = pairs[2 * k], gj = pairs[2 * k + 1];
-            const int r0 = items ? gi : gi / c.tr * c.tr, c0 = items ? gj : gj / c.tc * c.tc;
+            const int r0 = gi / c.tr * c.tr, c0 = gj / c.tc * c.tc;
             const long long tk = (long long)r0 * e->D + c0;
             auto it = key.find(tk);
             int idx;
@@ -585,7 +552,6 @@ int parent_engine_pair_counts(parent_engine* e, int method, int n_pairs_req, con
                 key[tk] = idx;
                 tl.push_back(r0);
                 tl.push_back(c0);
-                if (items) tl.push_back(c0 + 1);
                 per_tile.emplace_back();
             } else {
                 idx = it->second;
@@ -597,7 +563,7 @@ int parent_engine_pair_counts(parent_engine* e, int method, int n_pairs_req, con
             for (int k : per_tile[t]) {
                 const int gi = pairs[2 * k], gj = pairs[2 * k + 1];
                 const int r0 = tl[stride * t], c0 = tl[stride * t + 1];
-                entries.push_back(items ? 0 : (gi - r0) * c.tc + (gj - c0));
+                entries.push_back((gi - r0) * c.tc + (gj - c0));
                 entries.push_back((int)(off[k] >> 31));
                 entries.push_back((int)(off[k] & 0x7FFFFFFF));
             }

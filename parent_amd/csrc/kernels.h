@@ -51,19 +51,12 @@ struct AtomicConfig {
 };
 int atomic_config_count();
 AtomicConfig atomic_config(int id);
-int atomic_kind(int id);  // 0 plain LDS, 1 sliced, 2 persistent sliced, 3 row-stationary (work items)
+int atomic_kind(int id);  // 0 plain LDS histograms, 1 sliced (bank-conflict-free) histograms
 size_t atomic_lds_bytes(int id);
 bool atomic_fits(int id, int bmax, size_t lds_max);  // histograms of bmax x bmax fit its slots and the LDS
 hipError_t launch_pairs_atomic(const PairInputs& in, int config_id, const int* tiles, int ntiles,
                                const int* dump_begin, const int* dump_entries, int* dump_out,
                                hipStream_t s);
-
-// row-stationary sliced kernel: items [n][3] = (row dof, first column, end
-// column), columns > row; bins <= 50; frames <= 14 * 16384
-constexpr int kRowstatMaxFrames = 14 * 16384;
-hipError_t launch_pairs_rowstat(const PairInputs& in, int niter_max, const int* items, int nitems,
-                                const int* dump_begin, const int* dump_entries, int* dump_out,
-                                hipStream_t s);
 
 // device Prim (MIST)
 struct MistArgs {

@@ -460,286 +460,13 @@ __global__ __launch_bounds__(BLOCK, 1) void k_pairs_sliced(PairInputs in, const 
     }
 }
 
-// Persistent form of the sliced kernel for 1 x 1 tiles: a resident workgroup
-// walks a strided share of its XCD's run of tiles.  One pair's fill is only a
-// few iterations of 16 frames per thread, so the latency of its first index
-// loads and of the epilogue would otherwise be exposed on every pair; here the
-// next pair's first words are loaded before the epilogue, the epilogue clears
-// the sub-counters it reads, and n ln n is computed in registers.
-template <int BLOCK, int SLOTC>
-__global__ __launch_bounds__(BLOCK, 1) void k_pairs_sliced_persistent(PairInputs in, const int* __restrict__ tiles,
-                                                                   int ntiles,
-                                                                   const int* __restrict__ dump_begin,
-                                                                   const int* __restrict__ dump_entries,
-                                                                   int* __restrict__ dump_out) {
-    extern __shared__ unsigned hist[];
-    constexpr int kSlotW = SLOTC * 16;
-    constexpr int kWaves = BLOCK / 64;
-    __shared__ double sT[kWaves];
-    __shared__ int sOcc[kWaves];
-    __shared__ double nls[256];
-    const int D = in.D, N = in.N, nD = in.nD;
-    const unsigned s16 = threadIdx.x & 15;
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    for (int n = threadIdx.x; n < 256; n += BLOCK) nls[n] = in.nlogn[n <= N ? n : 0];
-    // XCD x = blockIdx % 8 owns tiles [x*per, (x+1)*per); its workgroups stride through them
-    const int per = (ntiles + 7) / 8;
-    const int x = blockIdx.x % 8, wx = blockIdx.x / 8, gx = gridDim.x / 8;
-    const int t_end = min(ntiles, (x + 1) * per);
-    int t = x * per + wx;
-    if (gx == 0 || t >= t_end) return;
-
-    {
-        uint4* h4 = reinterpret_cast<uint4*>(hist);
-        for (int i = threadIdx.x; i < kSlotW / 4; i += BLOCK) h4[i] = make_uint4(0, 0, 0, 0);
-    }
-    const int per_iter = BLOCK * 16;
-    const int niter = (N + per_iter - 1) / per_iter;
-    auto load = [&](int gi, int gj, int it, uint4& R, uint4& C) {
-        const int f = it * per_iter + threadIdx.x * 16;
-        if (f < in.Np) {
-            R = *reinterpret_cast<const uint4*>(in.bins + (size_t)gi * in.Np + f);
-            C = *reinterpret_cast<const uint4*>(in.bins + (size_t)gj * in.Np + f);
-        } else {
-            R = make_uint4(0, 0, 0, 0);
-            C = R;
-        }
-    };
-    int gi = tiles[2 * t], gj = tiles[2 * t + 1];
-    uint4 Rn, Cn;
-    load(gi, gj, 0, Rn, Cn);
-    __syncthreads();
-    for (; t < t_end; t += gx) {
-        const int Bi = in.B.of(dof_type(nD, gi)), Bj = in.B.of(dof_type(nD, gj));
-        const unsigned B16 = (unsigned)Bj * 16u;
-        const int tn = t + gx;
-        const int gin = tn < t_end ? tiles[2 * tn] : gi, gjn = tn < t_end ? tiles[2 * tn + 1] : gj;
-        for (int it = 0; it < niter; it++) {
-            const uint4 R = Rn, C = Cn;
-            if (it + 1 < niter) load(gi, gj, it + 1, Rn, Cn);
-            else load(gin, gjn, 0, Rn, Cn);
-            const int f = it * per_iter + threadIdx.x * 16;
-            if (f + 16 <= N) {
-#pragma unroll
-                for (int e = 0; e < 16; e++)
-                    atomicAdd(&hist[__umul24(byte_of(R, e), B16) + (byte_of(C, e) << 4) + s16], 1u);
-            } else {
-                const int nvalid = max(0, N - f);
-                for (int e = 0; e < nvalid; e++)
-                    atomicAdd(&hist[__umul24(byte_of(R, e), B16) + (byte_of(C, e) << 4) + s16], 1u);
-            }
-        }
-        __syncthreads();
-        const int cells = Bi * Bj;
-        if (dump_out) {
-            const int e0 = dump_begin[t], e1 = dump_begin[t + 1];
-            for (int e = e0; e < e1; e++) {
-                const long long off = ((long long)dump_entries[3 * e + 1] << 31) | (long long)dump_entries[3 * e + 2];
-                for (int i = threadIdx.x; i < cells; i += BLOCK) {
-                    const uint4* q = reinterpret_cast<const uint4*>(hist + i * 16);
-                    unsigned n = 0;
-#pragma unroll
-                    for (int k = 0; k < 4; k++) n += q[k].x + q[k].y + q[k].z + q[k].w;
-                    dump_out[off + i] = (int)n;
-                }
-            }
-            __syncthreads();
-        }
-        double T = 0.0;
-        int occ = 0;
-        for (int i = threadIdx.x; i < cells; i += BLOCK) {
-            uint4* q = reinterpret_cast<uint4*>(hist + i * 16);
-            unsigned n = 0;
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const uint4 v = q[k];
-                n += v.x + v.y + v.z + v.w;
-                q[k] = make_uint4(0, 0, 0, 0);
-            }
-            if (n) {
-                T += n < 256u ? nls[n] : in.nlogn[n];
-                occ++;
-            }
-        }
-        T = wave_sum(T);
-        occ = wave_sum(occ);
-        if (lane == 0) {
-            sT[wid] = T;
-            sOcc[wid] = occ;
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double tt = 0.0;
-            int oo = 0;
-            for (int w = 0; w < kWaves; w++) {
-                tt += sT[w];
-                oo += sOcc[w];
-            }
-            in.s2d[par_offset(nD, gi, gj)] = ((in.logN + in.c2[gi]) + in.c2[gj] + (oo - 1.0) / (2.0 * N)) - tt / N;
-        }
-        gi = gin;
-        gj = gjn;
-    }
-}
-
-// Row-stationary sliced kernel (1 x 1 pairs, one 1024-thread workgroup per CU).
-// Work items (row dof i, columns [j0, j1)) are ordered column-segment major,
-// so the workgroups running together stream the same column-dof rows through
-// L2.  The row's bin indices stay in registers for the whole item (NITER
-// uint4 per thread, frames tid*16 + it*16384), so a pair costs one column
-// stream (1 byte per update).  Sub-counters are never cleared between pairs:
-// each thread keeps the previous sums of the cell quarters it reads in the
-// epilogue and takes the difference (mod 2^32, exact for any per-pair count
-// < 2^32); they are cleared only when the histogram shape (bins_i, bins_j)
-// changes.  The epilogue reads a cell's 64 bytes with 4 lanes (conflict-free
-// ds_read_b128) and sums the quarters with lane shuffles.
-template <int NITER, bool PREFETCH>
-__global__ __launch_bounds__(1024, 1) void k_pairs_rowstat(PairInputs in, const int* __restrict__ items, int nitems,
-                                                        const int* __restrict__ dump_begin,
-                                                        const int* __restrict__ dump_entries,
-                                                        int* __restrict__ dump_out) {
-    constexpr int kBlock = 1024, kSlotC = 2500, kSlotW = kSlotC * 16, kWaves = kBlock / 64;
-    constexpr int kPerIter = kBlock * 16;
-    constexpr int kCellsPerPass = kBlock / 4;
-    constexpr int kK = (kSlotC + kCellsPerPass - 1) / kCellsPerPass;
-    extern __shared__ unsigned hist[];
-    __shared__ double nls[256];
-    __shared__ double sT[kWaves];
-    __shared__ int sO[kWaves];
-    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    const unsigned s16 = tid & 15;
-    const int q = tid & 3, cbase = tid >> 2;
-    const int D = in.D, N = in.N, nD = in.nD;
-    const int niter = (N + kPerIter - 1) / kPerIter;  // <= NITER (host checks)
-    for (int n = tid; n < 256; n += kBlock) nls[n] = in.nlogn[n < N ? n : 0];
-    unsigned prev[kK];
-#pragma unroll
-    for (int k = 0; k < kK; k++) prev[k] = 0;
-    int curBi = -1, curBj = -1;
-
-    auto load_row = [&](const uint8_t* base, uint4* X) {
-#pragma unroll
-        for (int it = 0; it < NITER; it++) {
-            const int f = it * kPerIter + tid * 16;
-            X[it] = (it < niter && f < in.Np) ? *reinterpret_cast<const uint4*>(base + f) : make_uint4(0, 0, 0, 0);
-        }
-    };
-    for (int item = blockIdx.x; item < nitems; item += gridDim.x) {
-        const int i = items[3 * item], j0 = items[3 * item + 1], j1 = items[3 * item + 2];
-        const int Bi = in.B.of(dof_type(nD, i));
-        uint4 R[NITER], C[NITER];
-        load_row(in.bins + (size_t)i * in.Np, R);
-        load_row(in.bins + (size_t)j0 * in.Np, C);
-        for (int j = j0; j < j1; j++) {
-            const int Bj = in.B.of(dof_type(nD, j));
-            if (Bi != curBi || Bj != curBj) {
-                __syncthreads();
-                uint4* h4 = reinterpret_cast<uint4*>(hist);
-                for (int w = tid; w < kSlotW / 4; w += kBlock) h4[w] = make_uint4(0, 0, 0, 0);
-#pragma unroll
-                for (int k = 0; k < kK; k++) prev[k] = 0;
-                curBi = Bi;
-                curBj = Bj;
-                __syncthreads();
-            }
-            // byte address of sub-counter (cell a*Bj + b, slice s) = a*64*Bj + b*64 + 4*s
-            const unsigned B64 = (unsigned)Bj * 64u, s4 = s16 * 4u;
-            char* hb = reinterpret_cast<char*>(hist);
-#pragma unroll
-            for (int it = 0; it < NITER; it++) {
-                if (it >= niter) break;
-                const int f = it * kPerIter + tid * 16;
-                if (f + 16 <= N) {
-#pragma unroll
-                    for (int e = 0; e < 16; e++)
-                        atomicAdd(reinterpret_cast<unsigned*>(hb + (__umul24(byte_of(R[it], e), B64) + (byte_of(C[it], e) << 6) + s4)), 1u);
-                } else {
-                    const int nvalid = max(0, N - f);
-                    for (int e = 0; e < nvalid; e++)
-                        atomicAdd(reinterpret_cast<unsigned*>(hb + (__umul24(byte_of(R[it], e), B64) + (byte_of(C[it], e) << 6) + s4)), 1u);
-                }
-            }
-            if (PREFETCH && j + 1 < j1) load_row(in.bins + (size_t)(j + 1) * in.Np, C);
-            __syncthreads();
-            // epilogue: 4 lanes per cell, each reads one 16-byte quarter
-            const int cells = Bi * Bj;
-            double T = 0.0;
-            int occ = 0;
-            long long doff = -1;
-            if (dump_out) {
-                for (int e = dump_begin[item]; e < dump_begin[item + 1]; e++)
-                    if (dump_entries[3 * e] == j - j0)
-                        doff = ((long long)dump_entries[3 * e + 1] << 31) | (long long)dump_entries[3 * e + 2];
-            }
-#pragma unroll
-            for (int k = 0; k < kK; k++) {
-                const int c = cbase + k * kCellsPerPass;
-                unsigned qs = 0;
-                if (c < cells) {
-                    const uint4 v = *reinterpret_cast<const uint4*>(hist + c * 16 + q * 4);
-                    qs = v.x + v.y + v.z + v.w;
-                }
-                unsigned d = qs - prev[k];
-                prev[k] = qs;
-                d += __shfl_xor(d, 1, 64);
-                d += __shfl_xor(d, 2, 64);
-                if (q == 0 && c < cells) {
-                    if (doff >= 0) dump_out[doff + c] = (int)d;
-                    if (d) {
-                        T += d < 256u ? nls[d] : (double)d * log((double)d);
-                        occ++;
-                    }
-                }
-            }
-            T = wave_sum(T);
-            occ = wave_sum(occ);
-            if (lane == 0) {
-                sT[wid] = T;
-                sO[wid] = occ;
-            }
-            __syncthreads();
-            if (tid == 0) {
-                double tt = 0.0;
-                int oo = 0;
-                for (int w = 0; w < kWaves; w++) {
-                    tt += sT[w];
-                    oo += sO[w];
-                }
-                in.s2d[par_offset(nD, i, j)] = ((in.logN + in.c2[i]) + in.c2[j] + (oo - 1.0) / (2.0 * N)) - tt / N;
-            }
-            if (!PREFETCH && j + 1 < j1) load_row(in.bins + (size_t)(j + 1) * in.Np, C);
-        }
-    }
-}
-
-hipError_t launch_pairs_rowstat(const PairInputs& in, int niter_max, const int* items, int nitems,
-                                const int* dump_begin, const int* dump_entries, int* dump_out,
-                                hipStream_t s) {
-    const size_t lds = (size_t)2500 * 64;
-    int dev = 0, cus = 0;
-    hipError_t e;
-    if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
-    if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
-    const int grid = std::max(1, std::min(cus, nitems));
-    const void* fn = niter_max <= 7 ? reinterpret_cast<const void*>(&k_pairs_rowstat<7, true>)
-                                    : reinterpret_cast<const void*>(&k_pairs_rowstat<14, false>);
-    if ((e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)) != hipSuccess) return e;
-    if (niter_max <= 7)
-        k_pairs_rowstat<7, true><<<grid, 1024, lds, s>>>(in, items, nitems, dump_begin, dump_entries, dump_out);
-    else
-        k_pairs_rowstat<14, false><<<grid, 1024, lds, s>>>(in, items, nitems, dump_begin, dump_entries, dump_out);
-    return hipGetLastError();
-}
-
 namespace {
 
 // (kind, tile rows, tile columns, threads, slot size).  kind 0: plain LDS
 // histograms, slot size in words (2560 -> B <= 50, 5632 -> B <= 75,
 // 10240 -> B <= 101, 40960 -> B <= 202); kind 1: sliced histograms, slot size
 // in cells (64 bytes each; 2500 -> B <= 50, 1225 -> B <= 35, 900 -> B <= 30,
-// 400 -> B <= 20, 100 -> B <= 10); kind 2: persistent sliced, 1 x 1 tiles,
-// slot size in cells; kind 3: row-stationary sliced (k_pairs_rowstat), run
-// on work items instead of tiles.
+// 400 -> B <= 20, 100 -> B <= 10).
 #define PARENT_ATOMIC_VARIANTS(X)     \
     X(0, 0, 1, 1, 256, 2560)          \
     X(1, 0, 2, 2, 256, 2560)          \
@@ -771,15 +498,7 @@ namespace {
     X(27, 1, 1, 4, 1024, 400)         \
     X(28, 1, 1, 1, 256, 400)          \
     X(29, 1, 2, 2, 512, 100)          \
-    X(30, 1, 4, 4, 1024, 100)         \
-    X(31, 2, 1, 1, 1024, 2500)        \
-    X(32, 2, 1, 1, 512, 2500)         \
-    X(33, 2, 1, 1, 1024, 1225)        \
-    X(34, 2, 1, 1, 512, 900)          \
-    X(35, 2, 1, 1, 512, 400)          \
-    X(36, 2, 1, 1, 256, 400)          \
-    X(37, 2, 1, 1, 256, 100)          \
-    X(38, 3, 1, 1, 1024, 2500)
+    X(30, 1, 4, 4, 1024, 100)
 
 struct AtomicVariant {
     AtomicConfig cfg;
@@ -791,9 +510,7 @@ struct AtomicVariant {
 template <int KIND, int TR, int TC, int BLK, int SL>
 const void* atomic_fn() {
     if constexpr (KIND == 0) return reinterpret_cast<const void*>(&k_pairs_atomic<TR, TC, BLK, SL>);
-    else if constexpr (KIND == 1) return reinterpret_cast<const void*>(&k_pairs_sliced<TR, TC, BLK, SL>);
-    else if constexpr (KIND == 2) return reinterpret_cast<const void*>(&k_pairs_sliced_persistent<BLK, SL>);
-    else return reinterpret_cast<const void*>(&k_pairs_rowstat<7, true>);
+    else return reinterpret_cast<const void*>(&k_pairs_sliced<TR, TC, BLK, SL>);
 }
 
 template <int KIND, int TR, int TC, int BLK, int SL>
@@ -801,7 +518,7 @@ void launch_variant(dim3 grid, size_t lds, hipStream_t s, const PairInputs& in, 
                     int t1, const int* db, const int* de, int* dout) {
     if constexpr (KIND == 0)
         k_pairs_atomic<TR, TC, BLK, SL><<<grid, BLK, lds, s>>>(in, tiles, t0, t1, db, de, dout);
-    else if constexpr (KIND == 1)
+    else
         k_pairs_sliced<TR, TC, BLK, SL><<<grid, BLK, lds, s>>>(in, tiles, t0, t1, db, de, dout);
 }
 
@@ -817,7 +534,7 @@ AtomicConfig atomic_config(int id) { return kAtomicVariants[id].cfg; }
 int atomic_kind(int id) { return kAtomicVariants[id].kind; }
 size_t atomic_lds_bytes(int id) {
     const AtomicVariant& v = kAtomicVariants[id];
-    return (size_t)v.cfg.tr * v.cfg.tc * v.slot * (v.kind == 0 ? 4 : 64);
+    return (size_t)v.cfg.tr * v.cfg.tc * v.slot * (v.kind == 1 ? 64 : 4);
 }
 bool atomic_fits(int id, int bmax, size_t lds_max) {
     return (long long)bmax * bmax <= kAtomicVariants[id].slot && atomic_lds_bytes(id) <= lds_max;
@@ -830,29 +547,6 @@ hipError_t launch_pairs_atomic(const PairInputs& in, int id, const int* tiles, i
     const size_t lds = atomic_lds_bytes(id);
     hipError_t e = hipFuncSetAttribute(kAtomicVariants[id].fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    if (kAtomicVariants[id].kind == 2) {
-        // resident workgroups: as many per CU as the LDS and the 2048 threads allow
-        int dev = 0, cus = 0, lds_cu = 0;
-        if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
-        if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
-        if ((e = hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev)) != hipSuccess) return e;
-        const int blk = kAtomicVariants[id].cfg.block;
-        int per_cu = std::min<int>(2048 / blk, (int)(lds_cu / (lds + 256)));
-        if (per_cu < 1) per_cu = 1;
-        const int nwg = ((cus * per_cu + 7) / 8) * 8;
-        switch (id) {
-#define PARENT_PERSISTENT_CASE(ID, K, TR, TC, BLK, SL)                                                  \
-    case ID:                                                                                             \
-        if constexpr (K == 2)                                                                            \
-            k_pairs_sliced_persistent<BLK, SL><<<nwg, BLK, lds, s>>>(in, tiles, ntiles, dump_begin,     \
-                                                                     dump_entries, dump_out);            \
-        break;
-            PARENT_ATOMIC_VARIANTS(PARENT_PERSISTENT_CASE)
-#undef PARENT_PERSISTENT_CASE
-            default: return hipErrorInvalidValue;
-        }
-        return hipGetLastError();
-    }
     const int per_launch = 1 << 22;
     for (int t0 = 0; t0 < ntiles; t0 += per_launch) {
         const int t1 = t0 + per_launch < ntiles ? t0 + per_launch : ntiles;
