@@ -26,6 +26,10 @@ sys.path.insert(0, ROOT)
 WORKLOADS = {
     # name: (atoms, frames, bins)
     "protein_1500x100k_b50": (1500, 100_000, 50),
+    "protein_1500x100k_b5": (1500, 100_000, 5),
+    "protein_1500x100k_b10": (1500, 100_000, 10),
+    "protein_1500x100k_b20": (1500, 100_000, 20),
+    "protein_1500x100k_b30": (1500, 100_000, 30),
     "chain_100x10k_b50": (100, 10_000, 50),
 }
 INT8_PEAK_OPS = 5.0e15  # MI355X dense int8 MFMA

@@ -44,7 +44,7 @@ struct DevBuf {
 };
 
 struct TileList {
-    int config = -1, D = -1;
+    int tr = -1, tc = -1, D = -1;
     int ntiles = 0;
     DevBuf dev;
 };
@@ -155,15 +155,28 @@ std::vector<int> make_tiles(int D, int tr, int tc) {
     return t;
 }
 
-int ensure_tiles(parent_engine* e, int config) {
-    if (e->tiles.config == config && e->tiles.D == e->D) return PARENT_OK;
-    const AtomicConfig c = atomic_config(config);
-    std::vector<int> t = make_tiles(e->D, c.tr, c.tc);
+// Tile shape (row dofs x column dofs) of a tiled method's variant.
+void tile_shape(const parent_engine* e, int method, int variant, int* tr, int* tc) {
+    if (method == PARENT_METHOD_MFMA) {
+        *tr = *tc = mfma_tile_dofs(bmax2d(e));
+    } else {
+        const AtomicConfig c = atomic_config(variant);
+        *tr = c.tr;
+        *tc = c.tc;
+    }
+}
+
+int ensure_tiles(parent_engine* e, int method, int variant) {
+    int tr, tc;
+    tile_shape(e, method, variant, &tr, &tc);
+    if (e->tiles.tr == tr && e->tiles.tc == tc && e->tiles.D == e->D) return PARENT_OK;
+    std::vector<int> t = make_tiles(e->D, tr, tc);
     HIPCHK(e, e->tiles.dev.ensure(std::max<size_t>(t.size(), 3) * sizeof(int)));
     if (!t.empty())
         HIPCHK(e, hipMemcpyAsync(e->tiles.dev.p, t.data(), t.size() * sizeof(int), hipMemcpyHostToDevice, e->stream));
     HIPCHK(e, hipStreamSynchronize(e->stream));
-    e->tiles.config = config;
+    e->tiles.tr = tr;
+    e->tiles.tc = tc;
     e->tiles.D = e->D;
     e->tiles.ntiles = (int)(t.size() / 2);
     return PARENT_OK;
@@ -172,10 +185,26 @@ int ensure_tiles(parent_engine* e, int config) {
 // Largest tile configuration whose histograms fit the LDS of one workgroup.
 bool variant_fits(const parent_engine* e, int id) { return atomic_fits(id, bmax2d(e), (size_t)e->lds_max); }
 
-// one launch of atomic-method configuration `id` over a work list
-hipError_t launch_variant(parent_engine* e, const PairInputs& in, int id, const int* work, int nwork,
+bool mfma_fits(const parent_engine* e) {
+    return mfma_tile_dofs(bmax2d(e)) > 0 && mfma_lds_bytes() <= (size_t)e->lds_max;
+}
+
+// one launch of a tiled method's variant over a work list
+hipError_t launch_variant(parent_engine* e, const PairInputs& in, int method, int id, const int* work, int nwork,
                           const int* db, const int* de, int* dout) {
+    if (method == PARENT_METHOD_MFMA) return launch_pairs_mfma(in, bmax2d(e), work, nwork, db, de, dout, e->stream);
     return launch_pairs_atomic(in, id, work, nwork, db, de, dout, e->stream);
+}
+
+// pairs (gi < gj < D) covered by a work list of tr x tc tiles
+long long tile_pairs(int D, const std::vector<int>& t, int tr, int tc) {
+    long long n = 0;
+    for (size_t k = 0; k + 1 < t.size(); k += 2)
+        for (int gi = t[k]; gi < std::min(t[k] + tr, D); gi++) {
+            const int lo = std::max(t[k + 1], gi + 1), hi = std::min(t[k + 1] + tc, D);
+            if (hi > lo) n += hi - lo;
+        }
+    return n;
 }
 
 int choose_atomic_config(const parent_engine* e) {
@@ -196,8 +225,8 @@ int check_ready(parent_engine* e) {
 }
 
 // Rows sampled by the autotuner: 8 runs of 12 rows spread over the dof range.
-// 12 is a multiple of every tile height, so every configuration covers exactly
-// the same pairs (all (gi, gj > gi) with gi in the sampled rows).
+// Each configuration's rate is its own work list's pair count (tile heights
+// that do not divide 12 cover a few more rows) over its time.
 std::vector<std::pair<int, int>> tune_rows(int D) {
     std::vector<std::pair<int, int>> rows;
     for (int k = 0; k < 8; k++) {
@@ -208,10 +237,9 @@ std::vector<std::pair<int, int>> tune_rows(int D) {
     return rows;
 }
 
-std::vector<int> tune_work(int D, int config, const std::vector<std::pair<int, int>>& rows) {
+std::vector<int> tune_work(int D, int tr, int tc, const std::vector<std::pair<int, int>>& rows) {
     std::vector<int> t;
-    const AtomicConfig c = atomic_config(config);
-    for (auto& rr : rows) push_band(t, D, c.tr, c.tc, rr.first, rr.second);
+    for (auto& rr : rows) push_band(t, D, tr, tc, rr.first, rr.second);
     return t;
 }
 
@@ -232,29 +260,33 @@ int run_autotune(parent_engine* e) {
     e->last_tune.clear();
     const int D = e->D;
     const auto rows = tune_rows(D);
-    long long npairs = 0;
-    for (auto& rr : rows)
-        for (int r = rr.first; r < rr.second; r++) npairs += D - 1 - r;
-    const double updates = (double)npairs * e->N;
     PairInputs in = pair_inputs(e);
     DevBuf dt;
     int best_m = PARENT_METHOD_PLAIN, best_v = 0;
     double best_rate = -1;
-    for (int id = 0; id < atomic_config_count(); id++) {
-        const AtomicConfig c = atomic_config(id);
-        if (!variant_fits(e, id)) continue;
-        std::vector<int> t = tune_work(D, id, rows);
-        HIPCHK(e, dt.ensure(t.size() * sizeof(int)));
+    // candidates: every fitting atomic configuration, then the MFMA kernel
+    std::vector<std::pair<int, int>> cand;
+    for (int id = 0; id < atomic_config_count(); id++)
+        if (variant_fits(e, id)) cand.push_back({PARENT_METHOD_ATOMIC, id});
+    if (mfma_fits(e)) cand.push_back({PARENT_METHOD_MFMA, 0});
+    for (auto& mv : cand) {
+        const int m = mv.first, id = mv.second;
+        int tr, tc;
+        tile_shape(e, m, id, &tr, &tc);
+        const int block = m == PARENT_METHOD_MFMA ? 256 : atomic_config(id).block;
+        std::vector<int> t = tune_work(D, tr, tc, rows);
+        const double updates = (double)tile_pairs(D, t, tr, tc) * e->N;
+        HIPCHK(e, dt.ensure(std::max<size_t>(t.size(), 2) * sizeof(int)));
         HIPCHK(e, hipMemcpyAsync(dt.p, t.data(), t.size() * sizeof(int), hipMemcpyHostToDevice, e->stream));
         double ms = 0;
         const int nt = (int)(t.size() / 2);
-        if (int rc = time_launch(e, [&] { return launch_variant(e, in, id, dt.as<int>(), nt, nullptr, nullptr, nullptr); }, &ms))
+        if (int rc = time_launch(e, [&] { return launch_variant(e, in, m, id, dt.as<int>(), nt, nullptr, nullptr, nullptr); }, &ms))
             return rc;
-        parent_tune_entry te{PARENT_METHOD_ATOMIC, id, c.tr, c.tc, c.block, ms, updates, updates / (ms * 1e-3)};
+        parent_tune_entry te{m, id, tr, tc, block, ms, updates, updates / (ms * 1e-3)};
         e->last_tune.push_back(te);
         if (te.updates_per_s > best_rate) {
             best_rate = te.updates_per_s;
-            best_m = PARENT_METHOD_ATOMIC;
+            best_m = m;
             best_v = id;
         }
     }
@@ -275,6 +307,27 @@ int tuned_choice(parent_engine* e, int* method, int* variant) {
     const auto& t = e->tuned.back();
     *method = t.method;
     *variant = t.variant;
+    return PARENT_OK;
+}
+
+// The kernel a request runs on: AUTO takes the autotuner's choice, MFMA falls
+// back to the atomic kernels outside its bin range, ATOMIC falls back to the
+// plain kernel when no tile configuration fits the LDS.
+int resolve_method(parent_engine* e, int method, int* used, int* variant) {
+    *used = method;
+    *variant = 0;
+    if (method == PARENT_METHOD_AUTO) return tuned_choice(e, used, variant);
+    if (method == PARENT_METHOD_MFMA) {
+        if (mfma_fits(e)) return PARENT_OK;
+        *used = PARENT_METHOD_ATOMIC;
+    }
+    if (*used == PARENT_METHOD_ATOMIC) {
+        *variant = choose_atomic_config(e);
+        if (*variant < 0) {
+            *used = PARENT_METHOD_PLAIN;
+            *variant = 0;
+        }
+    }
     return PARENT_OK;
 }
 
@@ -437,23 +490,15 @@ int parent_engine_pairs(parent_engine* e, int method) {
         return fail(e, PARENT_E_UNSUPPORTED, "2D histogram does not fit in LDS");
     PairInputs in = pair_inputs(e);
     int used = method, variant = 0;
-    if (method == PARENT_METHOD_AUTO) {
-        if (int rc = tuned_choice(e, &used, &variant)) return rc;
-    } else if (method == PARENT_METHOD_MFMA) {
-        used = PARENT_METHOD_ATOMIC;
-    }
-    if (used == PARENT_METHOD_ATOMIC && method != PARENT_METHOD_AUTO) {
-        variant = choose_atomic_config(e);
-        if (variant < 0) used = PARENT_METHOD_PLAIN;
-    }
-    if (used == PARENT_METHOD_ATOMIC) {
-        if (int rc = ensure_tiles(e, variant)) return rc;
+    if (int rc = resolve_method(e, method, &used, &variant)) return rc;
+    if (used != PARENT_METHOD_PLAIN) {
+        if (int rc = ensure_tiles(e, used, variant)) return rc;
     }
     HIPCHK(e, hipEventRecord(e->ev[4], e->stream));
     if (used == PARENT_METHOD_PLAIN) {
         HIPCHK(e, launch_pairs_plain(in, nullptr, e->stream));
-    } else if (used == PARENT_METHOD_ATOMIC) {
-        HIPCHK(e, launch_variant(e, in, variant, e->tiles.dev.as<int>(), e->tiles.ntiles, nullptr, nullptr, nullptr));
+    } else if (used == PARENT_METHOD_ATOMIC || used == PARENT_METHOD_MFMA) {
+        HIPCHK(e, launch_variant(e, in, used, variant, e->tiles.dev.as<int>(), e->tiles.ntiles, nullptr, nullptr, nullptr));
     } else {
         return fail(e, PARENT_E_INVALID, "unknown method");
     }
@@ -523,9 +568,8 @@ int parent_engine_pair_counts(parent_engine* e, int method, int n_pairs_req, con
     DevBuf dpairs, doff, dout;
     HIPCHK(e, dout.ensure(total * 4));
     PairInputs in = pair_inputs(e);
-    int used = method == PARENT_METHOD_AUTO || method == PARENT_METHOD_MFMA ? PARENT_METHOD_ATOMIC : method;
-    int variant = used == PARENT_METHOD_ATOMIC ? choose_atomic_config(e) : 0;
-    if (used == PARENT_METHOD_ATOMIC && variant < 0) used = PARENT_METHOD_PLAIN;
+    int used = method, variant = 0;
+    if (int rc = resolve_method(e, method, &used, &variant)) return rc;
     int rc = PARENT_OK;
     if (used == PARENT_METHOD_PLAIN) {
         HIPCHK(e, dpairs.ensure((size_t)n_pairs_req * 8));
@@ -534,9 +578,10 @@ int parent_engine_pair_counts(parent_engine* e, int method, int n_pairs_req, con
         HIPCHK(e, hipMemcpyAsync(doff.p, off.data(), (size_t)n_pairs_req * 8, hipMemcpyHostToDevice, e->stream));
         CountsDump dump{n_pairs_req, dpairs.as<int>(), doff.as<long long>(), dout.as<int>()};
         HIPCHK(e, launch_pairs_plain(in, &dump, e->stream));
-    } else if (used == PARENT_METHOD_ATOMIC) {
+    } else if (used == PARENT_METHOD_ATOMIC || used == PARENT_METHOD_MFMA) {
         // the tiles holding the requested pairs, each pair as (slot, offset)
-        const AtomicConfig c = atomic_config(variant);
+        AtomicConfig c{};
+        tile_shape(e, used, variant, &c.tr, &c.tc);
         const int stride = 2;
         std::vector<std::vector<int>> per_tile;
         std::vector<int> tl;
@@ -576,7 +621,7 @@ int parent_engine_pair_counts(parent_engine* e, int method, int n_pairs_req, con
         HIPCHK(e, hipMemcpyAsync(dtiles.p, tl.data(), tl.size() * 4, hipMemcpyHostToDevice, e->stream));
         HIPCHK(e, hipMemcpyAsync(dbegin.p, begin.data(), begin.size() * 4, hipMemcpyHostToDevice, e->stream));
         HIPCHK(e, hipMemcpyAsync(dent.p, entries.data(), entries.size() * 4, hipMemcpyHostToDevice, e->stream));
-        HIPCHK(e, launch_variant(e, in, variant, dtiles.as<int>(), (int)(tl.size() / stride), dbegin.as<int>(), dent.as<int>(), dout.as<int>()));
+        HIPCHK(e, launch_variant(e, in, used, variant, dtiles.as<int>(), (int)(tl.size() / stride), dbegin.as<int>(), dent.as<int>(), dout.as<int>()));
         HIPCHK(e, hipStreamSynchronize(e->stream));
         dtiles.release();
         dbegin.release();
@@ -695,7 +740,8 @@ int parent_engine_set_variant(parent_engine* e, int method, int variant) {
 
 int parent_engine_variant_count(parent_engine* e, int method, int* count) {
     if (!e || !count) return PARENT_E_INVALID;
-    *count = method == PARENT_METHOD_ATOMIC ? atomic_config_count() : (method == PARENT_METHOD_PLAIN ? 1 : 0);
+    *count = method == PARENT_METHOD_ATOMIC ? atomic_config_count()
+                                            : (method == PARENT_METHOD_PLAIN || method == PARENT_METHOD_MFMA ? 1 : 0);
     return PARENT_OK;
 }
 

@@ -58,6 +58,13 @@ hipError_t launch_pairs_atomic(const PairInputs& in, int config_id, const int* t
                                const int* dump_begin, const int* dump_entries, int* dump_out,
                                hipStream_t s);
 
+// one-hot int8 MFMA kernel: square tiles of mfma_tile_dofs(bmax) dofs a side
+// (0 when the 2D bin count is outside the kernel's range)
+int mfma_tile_dofs(int bmax);
+size_t mfma_lds_bytes();
+hipError_t launch_pairs_mfma(const PairInputs& in, int bmax, const int* tiles, int ntiles, const int* dump_begin,
+                             const int* dump_entries, int* dump_out, hipStream_t s);
+
 // device Prim (MIST)
 struct MistArgs {
     int nD, V;

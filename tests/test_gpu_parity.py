@@ -167,6 +167,60 @@ def test_every_atomic_variant_bit_exact(engine):
         assert tested >= 1
 
 
+MFMA_CASES = [
+    ([5] * 6, 30, 3001),
+    ([10] * 6, 26, 4097),
+    ([2] * 6, 20, 1000),
+    ([64] * 6, 12, 1500),
+    ([3, 9, 12, 5, 7, 11], 24, 2500),   # mixed types: 12-row dof slots, per-type cells
+    ([20] * 6, 16, 17000),              # frames not a multiple of the 128-frame chunk
+    ([40] * 6, 14, 2000),
+]
+
+
+@pytest.mark.parametrize("bins,atoms,frames", MFMA_CASES)
+def test_mfma_counts_and_entropies(engine, bins, atoms, frames):
+    """One-hot MFMA path: every pair's histogram bit-exact against the plain
+    kernel, a pair sample against the oracle, S2D and MI within tolerance."""
+    pa = _pa()
+    print(f"mfma bins={bins} atoms={atoms} frames={frames}", flush=True)
+    vals, nD = _case(atoms, frames, 17)
+    engine.set_trajectory(vals)
+    engine.prepare(bins)
+    engine.pairs(pa.METHOD_MFMA)
+    st = engine.stats()
+    assert st["method"] == pa.METHOD_MFMA
+    s1, s2 = engine.s1d(), engine.s2d()
+    s1o, s2o, mno, mxo, _, shifted = _oracle_full(vals, nD, bins)
+    assert np.max(np.abs(s2 - s2o)) <= S_ABS
+    mi, mio = _mi(nD, s1, s2), _mi(nD, s1o, s2o)
+    err = np.abs(mi - mio)
+    assert np.all(err <= MI_REL * np.abs(mio) + MI_ABS), f"max MI err {err.max():.3e}"
+    D = 3 * nD + 3
+    gi, gj = np.triu_indices(D, 1)
+    pairs = list(zip(gi.tolist(), gj.tolist()))
+    got = engine.pair_counts(pairs, pa.METHOD_MFMA, bins[3:])
+    plain = engine.pair_counts(pairs, pa.METHOD_PLAIN, bins[3:])
+    for p, h, hp in zip(pairs, got, plain):
+        assert np.array_equal(h, hp), f"counts differ for pair {p}"
+        assert h.sum() == frames
+    for a, b in [(0, 1), (0, D - 1), (nD + 1, 2 * nD + 5), (D - 2, D - 1)]:
+        ta, tb = O.type_of(nD, a), O.type_of(nD, b)
+        ho = O.entropy2d(ta, shifted[a], mno[a], mxo[a], bins[3 + ta],
+                         tb, shifted[b], mno[b], mxo[b], bins[3 + tb], counts=True)[1]
+        assert np.array_equal(got[pairs.index((a, b))].astype(np.int64), ho)
+
+
+def test_mfma_out_of_range_falls_back(engine):
+    """Above 64 bins the MFMA request runs on the atomic kernels."""
+    pa = _pa()
+    vals, nD = _case(10, 1000, 2)
+    engine.set_trajectory(vals)
+    engine.prepare([100] * 6)
+    engine.pairs(pa.METHOD_MFMA)
+    assert engine.stats()["method"] == pa.METHOD_ATOMIC
+
+
 @pytest.mark.parametrize("atoms,frames,seed", [(12, 1000, 3), (40, 3000, 21)])
 def test_mist_identical(engine, atoms, frames, seed):
     print(f"mist atoms={atoms} frames={frames}", flush=True)
