@@ -35,6 +35,10 @@ WORKLOADS = {
 INT8_PEAK_OPS = 5.0e15  # MI355X dense int8 MFMA
 HBM_PEAK = 8.0e12
 LDS_PEAK_FALLBACK = {"conflict_free": 9.5e12, "random_2500": 5.2e12}  # lds_peak.x on MI355X
+# HBM bytes per launch of the pair kernel, from rocprofv3 FETCH_SIZE and WRITE_SIZE passes scaled
+# by the elementwise-kernel calibration (FETCH x 2.00, WRITE x 1.00):
+# profiles/pairs_atomic_protein_1500x100k_b50.md (65.6 GB read + 0.243 GB written).
+PROFILED_TRAFFIC = {("protein_1500x100k_b50", 1): 65.6e9 + 0.243e9}
 
 
 def log(msg):
@@ -185,6 +189,12 @@ def main():
                     "unit": "Gatomic/s", "frac": kernel_rate / peak["conflict_free"], "traffic": None,
                     "frac_of_random_scatter_peak": kernel_rate / peak["random_2500"],
                     "peak_source": peak["source"] + "; conflict-free ds_add_u32, 16 lanes/clk/CU"}
+    traffic = PROFILED_TRAFFIC.get((args.workload, st["method"]))
+    roofline["traffic"] = traffic
+    if traffic is not None:
+        roofline["hbm"] = {"achieved": traffic / (pair_avg / 1e3) / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                           "frac": traffic / (pair_avg / 1e3) / HBM_PEAK,
+                           "algorithmic_bytes": D * (-(-frames // 4096) * 4096) + P * 8}
     roofline.update({"kernel": "pairs", "kernel_ms": pair_avg,
                      "algorithmic_work_per_launch": f"{updates:.4e} pair-frame updates (1 LDS atomic each)",
                      "mfma_one_hot_ceiling_updates_per_s": mfma_ceiling})
