@@ -6,14 +6,25 @@ histograms with their entropy epilogue (the reference's "Calculation time"
 span, PARENT.cpp:498-1489).  The trajectory (synthetic, float32, frame-major)
 is uploaded to HBM before the timed region.
 
-Default workload: a 1,500-atom synthetic chain, 4,494 dofs (10,095,771 pairs)
-x 100,000 frames, 50 bins per dof for every type.
+Workloads (--workload):
+  protein_1500x100k_b50   default: 1,500-atom chain, 4,494 dofs (10,095,771 pairs)
+                          x 100,000 frames, 50 bins for every type
+  protein_1500x100k_bB    same at B = 5, 10, 20, 30 bins
+  chain_100x10k_b50       100-atom chain, 294 dofs x 10,000 frames (the small case)
+  large_5000x200k_b30     5,000-atom chain, 14,994 dofs (112,402,521 pairs) x 200,000 frames, 30 bins
+  sweep_1500x100k         the default trajectory at B in {5,10,20,30,40,50,75,100}: one step = one
+                          launch of the captured graph holding every set's prepare and pair kernels
+  batch64_b30             64 chains, atoms log-uniform in [50, 2000], frames log-uniform in
+                          [5k, 200k] (seed 4), 30 bins: one step = the pre-binning of every
+                          trajectory (timed on the device) + one pair launch over the merged queue
 
 Prints one JSON line.  Extra keys: "plain" (the one-pair-per-workgroup
-kernel on the same workload), "phases" and the CPU baseline.
+kernel on the same workload), "phases", "mist", "roofline" and the CPU
+baseline.
 """
 import argparse
 import json
+import math
 import os
 import subprocess
 import sys
@@ -31,7 +42,10 @@ WORKLOADS = {
     "protein_1500x100k_b20": (1500, 100_000, 20),
     "protein_1500x100k_b30": (1500, 100_000, 30),
     "chain_100x10k_b50": (100, 10_000, 50),
+    "large_5000x200k_b30": (5000, 200_000, 30),
 }
+SWEEP_BINS = [5, 10, 20, 30, 40, 50, 75, 100]
+BATCH_N, BATCH_BINS, BATCH_SEED = 64, 30, 4
 INT8_PEAK_OPS = 5.0e15  # MI355X dense int8 MFMA
 HBM_PEAK = 8.0e12
 LDS_PEAK_FALLBACK = {"conflict_free": 9.5e12, "random_2500": 5.2e12}  # lds_peak.x on MI355X
@@ -39,6 +53,7 @@ LDS_PEAK_FALLBACK = {"conflict_free": 9.5e12, "random_2500": 5.2e12}  # lds_peak
 # by the elementwise-kernel calibration (FETCH x 2.00, WRITE x 1.00):
 # profiles/pairs_atomic_protein_1500x100k_b50.md (65.6 GB read + 0.243 GB written).
 PROFILED_TRAFFIC = {("protein_1500x100k_b50", 1): 65.6e9 + 0.243e9}
+METRIC = "pair-frame updates/s, all-pairs MI (4,491 dofs,100k fr,50 bins): tuned vs plain"
 
 
 def log(msg):
@@ -92,34 +107,53 @@ def cpu_baseline(atoms_sample, frames, bins, nthreads):
     return None
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="protein_1500x100k_b50", choices=sorted(WORKLOADS))
-    ap.add_argument("--method", default="auto", choices=["auto", "plain", "atomic", "mfma"])
-    ap.add_argument("--plain-steps", type=int, default=1, help="timed steps of the plain kernel (0: skip)")
-    ap.add_argument("--cpu-sample-atoms", type=int, default=500)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--tune-report", action="store_true", help="print the autotuner's table")
-    ap.add_argument("--variant", type=int, default=-1, help="pin the atomic method's configuration")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (--gpus {args.gpus} requested); "
-              "the multi-GPU sharding is a design note, not a runnable path.", file=sys.stderr)
-        sys.exit(2)
+def roofline_of(pa, method, bins, updates, kernel_ms, workload, D, P, frames):
+    """Roofline of the dominant kernel (the pair kernel).  On the LDS-atomic
+    paths one pair-frame update is one ds_add_u32, so the bound is the LDS
+    atomic rate (measured: no published figure exists); the one-hot int8 MFMA
+    path needs 2 B^2 int8 ops per update against the int8 peak."""
+    rate = updates / (kernel_ms / 1e3)
+    mfma_ceiling = INT8_PEAK_OPS / (2.0 * bins * bins)
+    if method == pa.METHOD_MFMA:
+        r = {"bound": "mfma", "achieved": 2.0 * bins * bins * rate / 1e12, "peak": INT8_PEAK_OPS / 1e12,
+             "unit": "TFLOP/s", "frac": 2.0 * bins * bins * rate / INT8_PEAK_OPS, "traffic": None,
+             "algorithmic_work_per_launch": f"{2.0 * bins * bins * updates:.4e} int8 ops (2 B^2 per update)"}
+    else:
+        peak = lds_peak()
+        r = {"bound": "lds_atomic", "achieved": rate / 1e9, "peak": peak["conflict_free"] / 1e9,
+             "unit": "Gatomic/s", "frac": rate / peak["conflict_free"], "traffic": None,
+             "frac_of_random_scatter_peak": rate / peak["random_2500"],
+             "peak_source": peak["source"] + "; conflict-free ds_add_u32, 16 lanes/clk/CU",
+             "algorithmic_work_per_launch": f"{updates:.4e} pair-frame updates (1 LDS atomic each)"}
+    traffic = PROFILED_TRAFFIC.get((workload, method))
+    r["traffic"] = traffic
+    if traffic is not None:
+        r["hbm"] = {"achieved": traffic / (kernel_ms / 1e3) / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                    "frac": traffic / (kernel_ms / 1e3) / HBM_PEAK,
+                    "algorithmic_bytes": D * (-(-frames // 4096) * 4096) + P * 8}
+    r.update({"kernel": "pairs", "kernel_ms": kernel_ms, "mfma_one_hot_ceiling_updates_per_s": mfma_ceiling})
+    return r
 
-    import numpy as np
-    import parent_amd as pa
 
+def emit(args, value, ms_step, config, extra):
+    out = {"metric": METRIC, "value": value, "unit": "pair-frame updates/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "u8", "data": "synthetic", "config": config}
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+
+
+def method_of(pa, name):
+    return {"auto": pa.METHOD_AUTO, "plain": pa.METHOD_PLAIN, "atomic": pa.METHOD_ATOMIC,
+            "mfma": pa.METHOD_MFMA}[name]
+
+
+def run_single(args, pa, nthreads):
     atoms, frames, bins = WORKLOADS[args.workload]
     D = 3 * atoms - 6
     P = D * (D - 1) // 2
     updates = P * frames
-    method = {"auto": pa.METHOD_AUTO, "plain": pa.METHOD_PLAIN, "atomic": pa.METHOD_ATOMIC,
-              "mfma": pa.METHOD_MFMA}[args.method]
-    nthreads = min(16, len(os.sched_getaffinity(0)))
+    method = method_of(pa, args.method)
     t0 = time.time()
     vals = pa.synth_values(atoms, frames, 2, nthreads)
     log(f"synthetic trajectory {vals.shape} in {time.time() - t0:.1f} s")
@@ -157,9 +191,14 @@ def main():
             f"method {st['method']} variant {st['kernel_variant']})")
     st = eng.stats()
     ms_step = 1e3 * sum(times) / len(times)
-    value = updates / (ms_step / 1e3)
     pair_avg = sum(pair_ms) / len(pair_ms)
     kernel_rate = updates / (pair_avg / 1e3)
+
+    mist = None
+    if not args.no_mist:
+        _, _, _, tot = eng.mist()
+        mist = {"ms": eng.stats()["ms_mist"], "vertices": D, "total_configurational_entropy": float(tot[-1])}
+        log(f"mist: {mist['ms']:.1f} ms, total {tot[-1]:.6f}")
 
     plain = None
     if args.plain_steps > 0 and method != pa.METHOD_PLAIN:
@@ -172,60 +211,158 @@ def main():
             pt.append(time.perf_counter() - t)
             log(f"plain step: {pt[-1] * 1e3:.1f} ms (pairs {eng.stats()['ms_pairs']:.2f} ms)")
         pms = 1e3 * sum(pt) / len(pt)
-        plain = {"value": updates / (pms / 1e3), "ms_per_step": pms, "speedup": (updates / (ms_step / 1e3)) / (updates / (pms / 1e3))}
+        plain = {"value": updates / (pms / 1e3), "ms_per_step": pms,
+                 "speedup": (updates / (ms_step / 1e3)) / (updates / (pms / 1e3))}
 
-    # Roofline of the dominant kernel (the pair kernel).  On the LDS-atomic
-    # paths one pair-frame update is one ds_add_u32, so the bound is the LDS
-    # atomic rate (measured: no published figure exists); the one-hot int8
-    # MFMA formulation would need 2*B^2 ops per update, i.e. at most
-    # INT8_PEAK / (2 B^2) updates/s, which is reported beside it.
-    peak = lds_peak()
-    mfma_ceiling = INT8_PEAK_OPS / (2.0 * bins * bins)
-    if st["method"] == pa.METHOD_MFMA:
-        roofline = {"bound": "mfma", "achieved": 2.0 * bins * bins * kernel_rate / 1e12, "peak": INT8_PEAK_OPS / 1e12,
-                    "unit": "TFLOP/s", "frac": 2.0 * bins * bins * kernel_rate / INT8_PEAK_OPS, "traffic": None}
-    else:
-        roofline = {"bound": "lds_atomic", "achieved": kernel_rate / 1e9, "peak": peak["conflict_free"] / 1e9,
-                    "unit": "Gatomic/s", "frac": kernel_rate / peak["conflict_free"], "traffic": None,
-                    "frac_of_random_scatter_peak": kernel_rate / peak["random_2500"],
-                    "peak_source": peak["source"] + "; conflict-free ds_add_u32, 16 lanes/clk/CU"}
-    traffic = PROFILED_TRAFFIC.get((args.workload, st["method"]))
-    roofline["traffic"] = traffic
-    if traffic is not None:
-        roofline["hbm"] = {"achieved": traffic / (pair_avg / 1e3) / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                           "frac": traffic / (pair_avg / 1e3) / HBM_PEAK,
-                           "algorithmic_bytes": D * (-(-frames // 4096) * 4096) + P * 8}
-    roofline.update({"kernel": "pairs", "kernel_ms": pair_avg,
-                     "algorithmic_work_per_launch": f"{updates:.4e} pair-frame updates (1 LDS atomic each)",
-                     "mfma_one_hot_ceiling_updates_per_s": mfma_ceiling})
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(args.cpu_sample_atoms, frames, bins, nthreads)
-    out = {
-        "metric": "pair-frame updates/s, all-pairs MI (4,491 dofs,100k fr,50 bins): tuned vs plain",
-        "value": value,
-        "unit": "pair-frame updates/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "u8",
-        "data": "synthetic",
-        "config": {"workload": args.workload, "atoms": atoms, "dofs": D, "pairs": P, "frames": frames,
-                   "bins": bins, "method": st["method"], "variant": st["kernel_variant"]},
-        "phases": {"prepare_ms": sum(prep_ms) / len(prep_ms), "pairs_ms": pair_avg,
-                   "pairs_kernel_updates_per_s": kernel_rate},
-        "plain": plain,
-        "roofline": roofline,
-        "cpu_baseline": cpu,
-    }
+    roofline = roofline_of(pa, st["method"], bins, updates, pair_avg, args.workload, D, P, frames)
+    cpu = None if args.no_cpu else cpu_baseline(args.cpu_sample_atoms, frames, bins, nthreads)
+    extra = {"phases": {"prepare_ms": sum(prep_ms) / len(prep_ms), "pairs_ms": pair_avg,
+                        "pairs_kernel_updates_per_s": kernel_rate},
+             "plain": plain, "mist": mist, "roofline": roofline, "cpu_baseline": cpu}
     if tune is not None:
-        out["tuning"] = tune
+        extra["tuning"] = tune
     eng.close()
-    print(json.dumps(out), flush=True)
+    emit(args, updates / (ms_step / 1e3), ms_step,
+         {"workload": args.workload, "atoms": atoms, "dofs": D, "pairs": P, "frames": frames, "bins": bins,
+          "method": st["method"], "variant": st["kernel_variant"]}, extra)
+
+
+def run_sweep(args, pa, nthreads):
+    atoms, frames = 1500, 100_000
+    D = 3 * atoms - 6
+    P = D * (D - 1) // 2
+    updates = P * frames * len(SWEEP_BINS)
+    method = method_of(pa, args.method)
+    vals = pa.synth_values(atoms, frames, 2, nthreads)
+    eng = pa.Engine(0)
+    eng.set_trajectory(vals)
+    del vals
+    t0 = time.time()
+    eng.sweep(SWEEP_BINS, method)  # first call: per-set prepare, tuning, tile lists, capture + one launch
+    log(f"sweep planned and captured in {time.time() - t0:.1f} s")
+    for _ in range(args.warmup):
+        eng.sweep(SWEEP_BINS, method)
+    times, graph_ms = [], []
+    for _ in range(args.steps):
+        t = time.perf_counter()
+        eng.sweep(SWEEP_BINS, method)
+        times.append(time.perf_counter() - t)
+        graph_ms.append(eng.stats()["ms_sweep"])
+        log(f"sweep step: {times[-1] * 1e3:.1f} ms (graph {graph_ms[-1]:.1f} ms)")
+    ms_step = 1e3 * sum(times) / len(times)
+    per_bin = []
+    for k, b in enumerate(SWEEP_BINS):
+        eng.select(k)
+        st = eng.stats()
+        s1 = eng.s1d()
+        entry = {"bins": b, "method": st["method"], "variant": st["kernel_variant"], "sum_s1d": float(s1.sum())}
+        if not args.no_mist:
+            _, mi, _, tot = eng.mist()
+            entry.update({"mist_ms": eng.stats()["ms_mist"], "mist_total_entropy": float(tot[-1]),
+                          "mist_sum_mi": float(mi.sum())})
+        per_bin.append(entry)
+        log(f"bins {b}: method {entry['method']} variant {entry['variant']} "
+            + (f"MIST total {entry['mist_total_entropy']:.6f}" if not args.no_mist else ""))
+    eng.close()
+    cpu = None if args.no_cpu else cpu_baseline(args.cpu_sample_atoms, frames, 50, nthreads)
+    if cpu:
+        cpu["sample"] += "; 50 bins stands for the sweep (CPU cost per update is flat in B)"
+    emit(args, updates / (ms_step / 1e3), ms_step,
+         {"workload": "sweep_1500x100k", "atoms": atoms, "dofs": D, "pairs": P, "frames": frames,
+          "bins": SWEEP_BINS, "method": args.method},
+         {"phases": {"graph_ms": sum(graph_ms) / len(graph_ms)}, "per_bin_count": per_bin, "cpu_baseline": cpu,
+          "roofline": None})
+
+
+def batch_shapes():
+    """64 (atoms, frames) pairs: log-uniform atoms in [50, 2000] and frames in
+    [5k, 200k], from a fixed splitmix64 stream (seed 4)."""
+    s = BATCH_SEED
+
+    def nxt():
+        nonlocal s
+        s = (s + 0x9E3779B97F4A7C15) & (2**64 - 1)
+        z = s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & (2**64 - 1)
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & (2**64 - 1)
+        return ((z ^ (z >> 31)) >> 11) / float(1 << 53)
+    out = []
+    for _ in range(BATCH_N):
+        a = int(round(math.exp(math.log(50) + nxt() * (math.log(2000) - math.log(50)))))
+        n = int(round(math.exp(math.log(5000) + nxt() * (math.log(200_000) - math.log(5000)))))
+        out.append((a, n))
+    return out
+
+
+def run_batch(args, pa, nthreads):
+    shapes = batch_shapes()
+    bins = BATCH_BINS
+    method = method_of(pa, args.method)
+    updates = sum((3 * a - 6) * (3 * a - 7) // 2 * n for a, n in shapes)
+    t0 = time.time()
+    b = pa.Batch([bins] * 6)
+    prep_ms = 0.0
+    for k, (a, n) in enumerate(shapes):
+        v = pa.synth_values(a, n, 1000 + k, nthreads)
+        b.add(v)
+        prep_ms += b.stats()["ms_prepare"]
+        del v
+    log(f"{len(shapes)} trajectories synthesised, uploaded and pre-binned in {time.time() - t0:.1f} s "
+        f"(device pre-binning {prep_ms:.1f} ms); {updates:.4e} pair-frame updates")
+    for _ in range(args.warmup):
+        b.run(method)
+    run_ms = []
+    for _ in range(args.steps):
+        b.run(method)
+        st = b.stats()
+        run_ms.append(st["ms_pairs"])
+        log(f"batch pair launch: {run_ms[-1]:.1f} ms (method {st['method']} variant {st['kernel_variant']})")
+    st = b.stats()
+    pair_avg = sum(run_ms) / len(run_ms)
+    ms_step = prep_ms + pair_avg
+    mist = []
+    if not args.no_mist:
+        for k in range(len(shapes)):
+            _, _, _, tot = b.mist(k)
+            mist.append(float(tot[-1]))
+    b.close()
+    roofline = roofline_of(pa, st["method"], bins, updates, pair_avg, "batch64_b30", 0, 0, 0)
+    cpu = None if args.no_cpu else cpu_baseline(args.cpu_sample_atoms, 100_000, bins, nthreads)
+    emit(args, updates / (ms_step / 1e3), ms_step,
+         {"workload": "batch64_b30", "trajectories": len(shapes), "bins": bins, "updates": updates,
+          "atoms_frames": shapes, "method": st["method"], "variant": st["kernel_variant"]},
+         {"phases": {"prepare_ms_total": prep_ms, "pairs_ms": pair_avg,
+                     "pairs_kernel_updates_per_s": updates / (pair_avg / 1e3)},
+          "mist_totals": mist, "roofline": roofline, "cpu_baseline": cpu})
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="protein_1500x100k_b50",
+                    choices=sorted(WORKLOADS) + ["sweep_1500x100k", "batch64_b30"])
+    ap.add_argument("--method", default="auto", choices=["auto", "plain", "atomic", "mfma"])
+    ap.add_argument("--plain-steps", type=int, default=1, help="timed steps of the plain kernel (0: skip)")
+    ap.add_argument("--cpu-sample-atoms", type=int, default=500)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mist", action="store_true")
+    ap.add_argument("--tune-report", action="store_true", help="print the autotuner's table")
+    ap.add_argument("--variant", type=int, default=-1, help="pin the atomic method's configuration")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (--gpus {args.gpus} requested); "
+              "the multi-GPU sharding is a design note, not a runnable path.", file=sys.stderr)
+        sys.exit(2)
+    import parent_amd as pa
+    nthreads = min(16, len(os.sched_getaffinity(0)))
+    if args.workload == "sweep_1500x100k":
+        run_sweep(args, pa, nthreads)
+    elif args.workload == "batch64_b30":
+        run_batch(args, pa, nthreads)
+    else:
+        run_single(args, pa, nthreads)
 
 
 if __name__ == "__main__":

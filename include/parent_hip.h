@@ -115,6 +115,8 @@ typedef struct {
     double pair_updates;  /* sum over pairs of frames */
     int method;           /* PARENT_METHOD_* actually used by the last parent_engine_pairs */
     int kernel_variant;   /* tile configuration id chosen for that method */
+    double ms_sweep;      /* last parent_engine_sweep: the graph launch (prepare + pairs of every set) */
+    int sweep_sets;       /* number of bin sets in that sweep */
 } parent_stats;
 
 /* One row of the autotuner's table. */
@@ -149,6 +151,23 @@ int parent_engine_prepare(parent_engine* eng, const int bins[6]);
  * PARENT.cpp:774-1479 (histogram fill at 802-804 and its 14 twins, epilogues
  * at 805-822, 838-855, 870-885, 1158-1175, 1190-1205, 1400-1413). */
 int parent_engine_pairs(parent_engine* eng, int method);
+
+/* Bin-count sweep over the resident trajectory: n_sets bin sets (bins: n_sets
+ * x 6 ints, each as for parent_engine_prepare), each pre-binned and paired
+ * with `method` (PARENT_METHOD_AUTO: the autotuner's choice per set, so small
+ * bin counts run on the MFMA kernel and large ones on the atomic kernels).
+ * The first call per (bin sets, method, trajectory) prepares every set,
+ * tunes, uploads the tile lists and captures all prepare and pair kernels of
+ * all sets into one hipGraph; every call launches that graph once
+ * (stats.ms_sweep).  Afterwards parent_engine_select(eng, k) makes set k the
+ * one that the read, pair_counts and mist calls use.  The reference runs
+ * PARENT.x once per bin count (run.sh:40); no counterpart of the batching. */
+int parent_engine_sweep(parent_engine* eng, int n_sets, const int* bins, int method);
+
+/* Selects the bin set (0 after prepare; 0..n_sets-1 after a sweep) that the
+ * read_*, pair_counts, mist and autotune calls act on; stats.method and
+ * stats.kernel_variant then name the pair kernel that set ran on. */
+int parent_engine_select(parent_engine* eng, int set);
 
 /* Copies results to the host (synchronises the engine's stream). */
 int parent_engine_read_s1d(parent_engine* eng, double* s1d /* D */);
@@ -207,6 +226,43 @@ int parent_engine_sync(parent_engine* eng);
 /* Phase timings of the last calls (HIP events on the engine's stream);
  * replaces the gettimeofday stopwatches of PARENT.cpp:95-97,498,1489,1501-1505. */
 int parent_engine_stats(parent_engine* eng, parent_stats* out);
+
+/* ------------------------------------------------------------------ */
+/* batch of independent trajectories (device)                          */
+/* ------------------------------------------------------------------ */
+
+/* Many trajectories of uneven dof and frame counts with one set of bins.
+ * The reference runs PARENT.x once per trajectory (run.sh:40); here each
+ * trajectory is pre-binned as it is added (PARENT.cpp:503-772, its values
+ * are then dropped from the device) and parent_batch_run computes every pair
+ * of every trajectory (PARENT.cpp:774-1479) in one launch over a merged tile
+ * queue ordered by cost.  No counterpart of the batching in the reference. */
+typedef struct parent_batch parent_batch;
+
+int parent_batch_create(int device, const int bins[6], parent_batch** out);
+void parent_batch_destroy(parent_batch* b);
+const char* parent_batch_error(const parent_batch* b);
+
+/* Adds one trajectory (frame-major values as for
+ * parent_engine_set_trajectory; a device pointer is only read during this
+ * call) and returns its index.  Fails like parent_engine_prepare on bad data. */
+int parent_batch_add(parent_batch* b, int n_dihedrals, int n_frames, int double_prec,
+                     const void* values, int on_device, int* index);
+
+/* All pairs of all added trajectories, one kernel launch (method as for
+ * parent_engine_pairs; the tile shape is chosen on the largest trajectory;
+ * PARENT_METHOD_PLAIN runs the tiled atomic kernel, as the plain kernel has
+ * no tile queue).  stats.ms_pairs is that launch. */
+int parent_batch_run(parent_batch* b, int method);
+
+/* S1D / S2D of trajectory `index` (either pointer may be NULL). */
+int parent_batch_read(parent_batch* b, int index, double* s1d, double* s2d);
+
+/* MIST of trajectory `index`, outputs as parent_engine_mist. */
+int parent_batch_mist(parent_batch* b, int index, int32_t* edges, double* mi, int32_t* left,
+                      double* totals);
+
+int parent_batch_stats(parent_batch* b, parent_stats* out);
 
 /* ------------------------------------------------------------------ */
 /* synthetic trajectories (no counterpart in the reference: test and   */

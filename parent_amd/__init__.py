@@ -18,6 +18,7 @@ from .api import (  # noqa: F401
     METHOD_MFMA,
     METHOD_PLAIN,
     Bat,
+    Batch,
     Engine,
     ParentError,
     dof_counts,

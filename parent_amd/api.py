@@ -64,6 +64,16 @@ def lib() -> C.CDLL:
         "parent_engine_set_variant": (i32, [p, i32, i32]),
         "parent_engine_variant_count": (i32, [p, i32, C.POINTER(i32)]),
         "parent_engine_stats": (i32, [p, p]),
+        "parent_engine_sweep": (i32, [p, i32, p, i32]),
+        "parent_engine_select": (i32, [p, i32]),
+        "parent_batch_create": (i32, [i32, p, C.POINTER(p)]),
+        "parent_batch_destroy": (None, [p]),
+        "parent_batch_error": (C.c_char_p, [p]),
+        "parent_batch_add": (i32, [p, i32, i32, i32, p, i32, C.POINTER(i32)]),
+        "parent_batch_run": (i32, [p, i32]),
+        "parent_batch_read": (i32, [p, i32, dp, dp]),
+        "parent_batch_mist": (i32, [p, i32, p, dp, p, dp]),
+        "parent_batch_stats": (i32, [p, p]),
         "parent_synth_values": (i32, [i32, i32, u64, p, i32]),
         "parent_synth_write_bat": (i32, [C.c_char_p, i32, i32, u64, i32]),
     }
@@ -84,7 +94,7 @@ class _TuneEntry(C.Structure):
 class _Stats(C.Structure):
     _fields_ = [("ms_upload", C.c_double), ("ms_prepare", C.c_double), ("ms_pairs", C.c_double),
                 ("ms_mist", C.c_double), ("pair_updates", C.c_double), ("method", C.c_int),
-                ("kernel_variant", C.c_int)]
+                ("kernel_variant", C.c_int), ("ms_sweep", C.c_double), ("sweep_sets", C.c_int)]
 
 
 def _ptr(a: np.ndarray) -> C.c_void_p:
@@ -246,6 +256,17 @@ class Engine:
     def pairs(self, method: int = METHOD_AUTO) -> None:
         self._check(lib().parent_engine_pairs(self._h, method))
 
+    def sweep(self, bin_sets, method: int = METHOD_AUTO) -> None:
+        """Prepare + all pairs for every bin set (rows of 6, or single counts) as one
+        captured hipGraph launch; parent_engine_sweep.  select(k) then picks set k."""
+        rows = [np.full(6, int(b), dtype=np.int32) if np.ndim(b) == 0 else np.asarray(b, dtype=np.int32)
+                for b in bin_sets]
+        arr = np.ascontiguousarray(np.stack(rows), dtype=np.int32)
+        self._check(lib().parent_engine_sweep(self._h, arr.shape[0], _ptr(arr), method))
+
+    def select(self, k: int) -> None:
+        self._check(lib().parent_engine_select(self._h, k))
+
     def s1d(self) -> np.ndarray:
         out = np.empty(self.D, dtype=np.float64)
         self._check(lib().parent_engine_read_s1d(self._h, _ptr(out)))
@@ -327,6 +348,78 @@ class Engine:
     def close(self) -> None:
         if getattr(self, "_h", None):
             lib().parent_engine_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Independent trajectories with one set of bins, all pairs in one launch
+    (parent_batch_*)."""
+
+    def __init__(self, bins, device: int = 0):
+        b = np.ascontiguousarray(bins, dtype=np.int32)
+        if b.size == 1:
+            b = np.full(6, int(b.reshape(-1)[0]), dtype=np.int32)
+        h = C.c_void_p()
+        _check_host(lib().parent_batch_create(device, _ptr(b), C.byref(h)))
+        self._h = h
+        self.n_dihedrals = []
+
+    def _check(self, rc: int) -> None:
+        if rc != 0:
+            raise ParentError(rc, lib().parent_batch_error(self._h).decode())
+
+    def add(self, values) -> int:
+        """Adds a frame-major [N][D] float32/float64 host array; returns its index."""
+        v = np.ascontiguousarray(values)
+        if v.dtype not in (np.float32, np.float64) or v.ndim != 2 or (v.shape[1] - 3) % 3:
+            raise ValueError("values must be [N][3*nD+3] float32 or float64")
+        nD = (v.shape[1] - 3) // 3
+        idx = C.c_int()
+        self._check(lib().parent_batch_add(self._h, nD, v.shape[0], int(v.dtype == np.float64), _ptr(v), 0,
+                                           C.byref(idx)))
+        self.n_dihedrals.append(nD)
+        return idx.value
+
+    def run(self, method: int = METHOD_AUTO) -> None:
+        self._check(lib().parent_batch_run(self._h, method))
+
+    def results(self, k: int):
+        """(s1d, s2d) of trajectory k."""
+        D = 3 * self.n_dihedrals[k] + 3
+        s1 = np.empty(D, dtype=np.float64)
+        s2 = np.empty(D * (D - 1) // 2, dtype=np.float64)
+        self._check(lib().parent_batch_read(self._h, k, _ptr(s1), _ptr(s2)))
+        return s1, s2
+
+    def mist(self, k: int):
+        V = 3 * self.n_dihedrals[k] + 3
+        edges = np.empty((V - 1, 4), dtype=np.int32)
+        mi = np.empty(V - 1, dtype=np.float64)
+        left = np.empty(V - 1, dtype=np.int32)
+        tot = np.empty(10, dtype=np.float64)
+        self._check(lib().parent_batch_mist(self._h, k, _ptr(edges), _ptr(mi), _ptr(left), _ptr(tot)))
+        return edges, mi, left, tot
+
+    def stats(self) -> dict:
+        s = _Stats()
+        self._check(lib().parent_batch_stats(self._h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in _Stats._fields_}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().parent_batch_destroy(self._h)
             self._h = None
 
     def __enter__(self):

@@ -4,14 +4,23 @@
 // frame-major values; prepare() turns it into dof-major uint8 bin indices
 // plus per-dof constants; pairs() runs one of the pair kernels over all
 // D(D-1)/2 dof pairs and leaves S2D in .par body order on the device.
+//
+// State is split in two: a Traj (the trajectory on the device) and one
+// BinSet per bin set (indices, per-dof constants, S1D/S2D and the kernel
+// chosen for it).  prepare()/pairs() use one BinSet; a sweep fills several
+// over the same trajectory and replays them as one captured hipGraph; a batch
+// (parent_batch_*) holds one (Traj, BinSet) per trajectory and runs all their
+// pair tiles through one launch.
 #include "../../include/parent_hip.h"
 #include "host_common.h"
 #include "kernels.h"
 
 #include <algorithm>
-#include <map>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <memory>
+#include <numeric>
 #include <string>
 #include <vector>
 
@@ -19,17 +28,35 @@ using namespace parent;
 
 namespace {
 
+// Owning device allocation (move-only).
 struct DevBuf {
     void* p = nullptr;
     size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) {
+        o.p = nullptr;
+        o.n = 0;
+    }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) {
+            release();
+            p = o.p;
+            n = o.n;
+            o.p = nullptr;
+            o.n = 0;
+        }
+        return *this;
+    }
+    ~DevBuf() { release(); }
     hipError_t ensure(size_t bytes) {
         if (bytes <= n && p) return hipSuccess;
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
+        release();
         if (bytes == 0) return hipSuccess;
         hipError_t e = hipMalloc(&p, bytes);
         if (e == hipSuccess) n = bytes;
+        else p = nullptr;
         return e;
     }
     void release() {
@@ -49,6 +76,31 @@ struct TileList {
     DevBuf dev;
 };
 
+// A trajectory resident on the device.
+struct Traj {
+    int nD = -1, N = 0, dp = 0, D = 0, Np = 0;
+    const void* values = nullptr;
+    DevBuf values_own;
+};
+
+// One bin set's results over a trajectory.
+struct BinSet {
+    int bins[6] = {0, 0, 0, 0, 0, 0};
+    bool prepared = false;   // indices and 1D results valid
+    bool s2d_valid = false;  // S2D valid
+    int ent_nD = -1;         // nD of the entropies in s1d/s2d
+    int method = PARENT_METHOD_PLAIN, variant = 0;  // kernel of the last pair pass
+    DevBuf flags, vmin, vmax, modfit, bins2, bins1, s1d, c2, jtab, s2d;
+    TileList tiles;
+};
+
+// Scratch of the prepare and MIST kernels, shared by the bin sets of a stream
+// (they run one after another on it).
+struct Scratch {
+    DevBuf emin, emax, mfhist, nlogn;
+    DevBuf mkey, mupar, minpos, mstate, medges, mmi, mleft, mflags;
+};
+
 }  // namespace
 
 struct parent_engine {
@@ -58,22 +110,12 @@ struct parent_engine {
     std::string err;
     int lds_max = 65536;
 
-    // trajectory
-    int nD = -1, N = 0, dp = 0, D = 0, Np = 0;
-    const void* values = nullptr;
-    DevBuf values_own;
-
-    // prepare outputs
-    int bins[6] = {0, 0, 0, 0, 0, 0};
-    bool prepared = false;
-    bool s2d_valid = false;
-    DevBuf emin, emax, mfhist, flags, vmin, vmax, modfit, bins2, bins1, s1d, c2, jtab, nlogn, s2d;
-
-    // MIST scratch
-    DevBuf mkey, mupar, minpos, mstate, medges, mmi, mleft;
-    int ent_nD = -1;  // nD of the entropies currently in s1d/s2d
-
-    TileList tiles;
+    Traj traj;
+    Scratch scratch;
+    // sets[0] belongs to prepare()/pairs(); a sweep fills sets[0..n-1]; the
+    // readers, pair_counts and mist act on sets[active]
+    std::vector<std::unique_ptr<BinSet>> sets;
+    int active = 0;
     parent_stats stats{};
 
     // autotuner: the fastest pair-kernel configuration per problem key
@@ -84,6 +126,25 @@ struct parent_engine {
     std::vector<Tuned> tuned;
     std::vector<parent_tune_entry> last_tune;
     int forced_atomic = -1;  // parent_engine_set_variant
+
+    // the captured sweep
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    std::vector<int> graph_key;
+    long long traj_serial = 0;  // bumped by every set_trajectory
+};
+
+struct parent_batch {
+    parent_engine* eng = nullptr;
+    int bins[6] = {0, 0, 0, 0, 0, 0};
+    struct Item {
+        Traj traj;
+        BinSet set;
+    };
+    std::vector<std::unique_ptr<Item>> items;
+    DevBuf queue, descs;
+    int queue_items = -1, queue_method = -1, queue_variant = -1, ntiles = 0;
+    int method = PARENT_METHOD_PLAIN, variant = 0;
 };
 
 namespace {
@@ -109,27 +170,115 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
     return ms;
 }
 
-Bins3 bins2d(const parent_engine* e) { return Bins3{e->bins[3], e->bins[4], e->bins[5]}; }
-Bins3 bins1d(const parent_engine* e) { return Bins3{e->bins[0], e->bins[1], e->bins[2]}; }
+Bins3 bins2d(const BinSet& s) { return Bins3{s.bins[3], s.bins[4], s.bins[5]}; }
+Bins3 bins1d(const BinSet& s) { return Bins3{s.bins[0], s.bins[1], s.bins[2]}; }
+int bmax2d(const BinSet& s) { return std::max(s.bins[3], std::max(s.bins[4], s.bins[5])); }
 
-int bmax2d(const parent_engine* e) { return std::max(e->bins[3], std::max(e->bins[4], e->bins[5])); }
+BinSet& cur(parent_engine* e) { return *e->sets[e->active]; }
 
-PairInputs pair_inputs(parent_engine* e) {
+int check_bins(parent_engine* e, const int* bins) {
+    for (int k = 0; k < 6; k++) {
+        if (bins[k] < 1) return fail(e, PARENT_E_INVALID, "number of bins < 1");
+        if (bins[k] > kMaxBins) return fail(e, PARENT_E_UNSUPPORTED, "more than 255 bins per dof is not supported");
+    }
+    return PARENT_OK;
+}
+
+PairInputs pair_inputs(const Traj& t, BinSet& s) {
     PairInputs in{};
-    in.bins = e->bins2.as<uint8_t>();
-    in.nD = e->nD;
-    in.D = e->D;
-    in.N = e->N;
-    in.Np = e->Np;
-    in.B = bins2d(e);
-    in.vmin = e->vmin.as<double>();
-    in.vmax = e->vmax.as<double>();
-    in.jtab = e->jtab.as<double>();
-    in.c2 = e->c2.as<double>();
-    in.nlogn = e->nlogn.as<double>();
-    in.logN = log((double)e->N);
-    in.s2d = e->s2d.as<double>();
+    in.bins = s.bins2.as<uint8_t>();
+    in.nD = t.nD;
+    in.D = t.D;
+    in.N = t.N;
+    in.Np = t.Np;
+    in.B = bins2d(s);
+    in.vmin = s.vmin.as<double>();
+    in.vmax = s.vmax.as<double>();
+    in.jtab = s.jtab.as<double>();
+    in.c2 = s.c2.as<double>();
+    in.logN = log((double)t.N);
+    in.s2d = s.s2d.as<double>();
     return in;
+}
+
+// Device buffers of bin set s over trajectory t, and the shared scratch.
+int alloc_set(parent_engine* e, const Traj& t, BinSet& s) {
+    const int D = t.D, nD = t.nD;
+    const bool need1 = s.bins[0] != s.bins[3] || s.bins[1] != s.bins[4] || s.bins[2] != s.bins[5];
+    Scratch& x = e->scratch;
+    HIPCHK(e, x.emin.ensure(D * 8));
+    HIPCHK(e, x.emax.ensure(D * 8));
+    HIPCHK(e, x.mfhist.ensure((size_t)nD * kModfitBins * 4));
+    HIPCHK(e, x.nlogn.ensure(((size_t)t.N + 1) * 8));
+    HIPCHK(e, s.flags.ensure(64));
+    HIPCHK(e, s.vmin.ensure(D * 8));
+    HIPCHK(e, s.vmax.ensure(D * 8));
+    HIPCHK(e, s.modfit.ensure(D * 8));
+    HIPCHK(e, s.bins2.ensure((size_t)D * t.Np));
+    if (need1) HIPCHK(e, s.bins1.ensure((size_t)D * t.Np));
+    else s.bins1.release();
+    HIPCHK(e, s.s1d.ensure(D * 8));
+    HIPCHK(e, s.c2.ensure(D * 8));
+    HIPCHK(e, s.jtab.ensure((size_t)D * 256 * 8));
+    HIPCHK(e, s.s2d.ensure((size_t)std::max<long long>(n_pairs(nD), 1) * 8));
+    return PARENT_OK;
+}
+
+// Ranges, modfit, pre-binning and 1D entropies of set s: kernels only, no
+// host synchronisation (capturable).  The n ln n table of t.N goes to the
+// shared scratch.
+hipError_t enqueue_prepare(parent_engine* e, const Traj& t, BinSet& s) {
+    const bool need1 = s.bins1.p != nullptr;
+    PrepareArgs a{};
+    a.values = t.values;
+    a.double_prec = t.dp;
+    a.nD = t.nD;
+    a.D = t.D;
+    a.N = t.N;
+    a.Np = t.Np;
+    a.pi = acos(-1.0);  // PARENT.cpp:112
+    a.B1 = bins1d(s);
+    a.B2 = bins2d(s);
+    a.emin = e->scratch.emin.as<unsigned long long>();
+    a.emax = e->scratch.emax.as<unsigned long long>();
+    a.mfhist = e->scratch.mfhist.as<unsigned>();
+    a.flags = s.flags.as<int>();
+    a.vmin = s.vmin.as<double>();
+    a.vmax = s.vmax.as<double>();
+    a.modfit = s.modfit.as<double>();
+    a.bins2 = s.bins2.as<uint8_t>();
+    a.bins1 = need1 ? s.bins1.as<uint8_t>() : nullptr;
+    a.s1d = s.s1d.as<double>();
+    a.c2 = s.c2.as<double>();
+    a.jtab = s.jtab.as<double>();
+    a.nlogn = e->scratch.nlogn.as<double>();
+    return launch_prepare(a, e->stream);
+}
+
+// Reads set s's data flags (synchronises) and maps them to the reference's errors.
+int check_flags(parent_engine* e, BinSet& s) {
+    int flags = 0;
+    HIPCHK(e, hipMemcpyAsync(&flags, s.flags.p, sizeof(int), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    if (flags & kFlagNegBond) return fail(e, PARENT_E_DATA, "ERROR: a bond length is smaller than 0.0");
+    if (flags & kFlagRange)
+        return fail(e, PARENT_E_RANGE, "ERROR: a value falls outside its histogram range (angle >= pi or dihedral outside [0, 2pi+5e-9))");
+    return PARENT_OK;
+}
+
+// Eager prepare of set s: enqueue, time, check.
+int run_prepare(parent_engine* e, const Traj& t, BinSet& s) {
+    s.prepared = false;
+    s.s2d_valid = false;
+    if (int rc = alloc_set(e, t, s)) return rc;
+    HIPCHK(e, hipEventRecord(e->ev[2], e->stream));
+    HIPCHK(e, enqueue_prepare(e, t, s));
+    HIPCHK(e, hipEventRecord(e->ev[3], e->stream));
+    if (int rc = check_flags(e, s)) return rc;
+    e->stats.ms_prepare = elapsed(e->ev[2], e->ev[3]);
+    s.prepared = true;
+    s.ent_nD = t.nD;
+    return PARENT_OK;
 }
 
 // Tiles of a TR x TC configuration over the upper triangle: row blocks at
@@ -156,9 +305,9 @@ std::vector<int> make_tiles(int D, int tr, int tc) {
 }
 
 // Tile shape (row dofs x column dofs) of a tiled method's variant.
-void tile_shape(const parent_engine* e, int method, int variant, int* tr, int* tc) {
+void tile_shape(const BinSet& s, int method, int variant, int* tr, int* tc) {
     if (method == PARENT_METHOD_MFMA) {
-        *tr = *tc = mfma_tile_dofs(bmax2d(e));
+        *tr = *tc = mfma_tile_dofs(bmax2d(s));
     } else {
         const AtomicConfig c = atomic_config(variant);
         *tr = c.tr;
@@ -166,34 +315,40 @@ void tile_shape(const parent_engine* e, int method, int variant, int* tr, int* t
     }
 }
 
-int ensure_tiles(parent_engine* e, int method, int variant) {
+int ensure_tiles(parent_engine* e, const Traj& t, BinSet& s, int method, int variant) {
     int tr, tc;
-    tile_shape(e, method, variant, &tr, &tc);
-    if (e->tiles.tr == tr && e->tiles.tc == tc && e->tiles.D == e->D) return PARENT_OK;
-    std::vector<int> t = make_tiles(e->D, tr, tc);
-    HIPCHK(e, e->tiles.dev.ensure(std::max<size_t>(t.size(), 3) * sizeof(int)));
-    if (!t.empty())
-        HIPCHK(e, hipMemcpyAsync(e->tiles.dev.p, t.data(), t.size() * sizeof(int), hipMemcpyHostToDevice, e->stream));
+    tile_shape(s, method, variant, &tr, &tc);
+    if (s.tiles.tr == tr && s.tiles.tc == tc && s.tiles.D == t.D) return PARENT_OK;
+    std::vector<int> tl = make_tiles(t.D, tr, tc);
+    HIPCHK(e, s.tiles.dev.ensure(std::max<size_t>(tl.size(), 3) * sizeof(int)));
+    if (!tl.empty())
+        HIPCHK(e, hipMemcpyAsync(s.tiles.dev.p, tl.data(), tl.size() * sizeof(int), hipMemcpyHostToDevice, e->stream));
     HIPCHK(e, hipStreamSynchronize(e->stream));
-    e->tiles.tr = tr;
-    e->tiles.tc = tc;
-    e->tiles.D = e->D;
-    e->tiles.ntiles = (int)(t.size() / 2);
+    s.tiles.tr = tr;
+    s.tiles.tc = tc;
+    s.tiles.D = t.D;
+    s.tiles.ntiles = (int)(tl.size() / 2);
     return PARENT_OK;
 }
 
-// Largest tile configuration whose histograms fit the LDS of one workgroup.
-bool variant_fits(const parent_engine* e, int id) { return atomic_fits(id, bmax2d(e), (size_t)e->lds_max); }
-
-bool mfma_fits(const parent_engine* e) {
-    return mfma_tile_dofs(bmax2d(e)) > 0 && mfma_lds_bytes() <= (size_t)e->lds_max;
+bool variant_fits(const parent_engine* e, const BinSet& s, int id) {
+    return atomic_fits(id, bmax2d(s), (size_t)e->lds_max);
 }
 
-// one launch of a tiled method's variant over a work list
-hipError_t launch_variant(parent_engine* e, const PairInputs& in, int method, int id, const int* work, int nwork,
-                          const int* db, const int* de, int* dout) {
-    if (method == PARENT_METHOD_MFMA) return launch_pairs_mfma(in, bmax2d(e), work, nwork, db, de, dout, e->stream);
-    return launch_pairs_atomic(in, id, work, nwork, db, de, dout, e->stream);
+bool mfma_fits(const parent_engine* e, const BinSet& s) {
+    return mfma_tile_dofs(bmax2d(s)) > 0 && mfma_lds_bytes() <= (size_t)e->lds_max;
+}
+
+PairInputs with_nlogn(parent_engine* e, PairInputs in) {
+    in.nlogn = e->scratch.nlogn.as<double>();
+    return in;
+}
+
+// one launch of a tiled method's variant over a tile queue
+hipError_t launch_tiled(parent_engine* e, const PairInputs& in, int bmax, int method, int id, const TileQueue& q,
+                        const int* db, const int* de, int* dout) {
+    if (method == PARENT_METHOD_MFMA) return launch_pairs_mfma(in, bmax, q, db, de, dout, e->stream);
+    return launch_pairs_atomic(in, id, q, db, de, dout, e->stream);
 }
 
 // pairs (gi < gj < D) covered by a work list of tr x tc tiles
@@ -207,21 +362,14 @@ long long tile_pairs(int D, const std::vector<int>& t, int tr, int tc) {
     return n;
 }
 
-int choose_atomic_config(const parent_engine* e) {
-    const int bmax = bmax2d(e);
-    if (e->forced_atomic >= 0) return variant_fits(e, e->forced_atomic) ? e->forced_atomic : -1;
+int choose_atomic_config(const parent_engine* e, const BinSet& s) {
+    if (e->forced_atomic >= 0) return variant_fits(e, s, e->forced_atomic) ? e->forced_atomic : -1;
     // fallback order when not autotuning: the best of the autotuner's tables
     // at 50 bins first, then configurations with larger histogram slots
     static const int order[] = {2, 3, 1, 9, 0, 11, 13, 10, 15, 16, 14, 18};
     for (int id : order)
-        if (variant_fits(e, id)) return id;
-    (void)bmax;
+        if (variant_fits(e, s, id)) return id;
     return -1;
-}
-
-int check_ready(parent_engine* e) {
-    if (!e->prepared) return fail(e, PARENT_E_INVALID, "parent_engine_prepare has not succeeded");
-    return PARENT_OK;
 }
 
 // Rows sampled by the autotuner: 8 runs of 12 rows spread over the dof range.
@@ -256,31 +404,31 @@ int time_launch(parent_engine* e, F launch, double* ms) {
     return PARENT_OK;
 }
 
-int run_autotune(parent_engine* e) {
+int run_autotune(parent_engine* e, const Traj& tj, BinSet& s) {
     e->last_tune.clear();
-    const int D = e->D;
+    const int D = tj.D;
     const auto rows = tune_rows(D);
-    PairInputs in = pair_inputs(e);
+    const PairInputs in = with_nlogn(e, pair_inputs(tj, s));
     DevBuf dt;
     int best_m = PARENT_METHOD_PLAIN, best_v = 0;
     double best_rate = -1;
     // candidates: every fitting atomic configuration, then the MFMA kernel
     std::vector<std::pair<int, int>> cand;
     for (int id = 0; id < atomic_config_count(); id++)
-        if (variant_fits(e, id)) cand.push_back({PARENT_METHOD_ATOMIC, id});
-    if (mfma_fits(e)) cand.push_back({PARENT_METHOD_MFMA, 0});
+        if (variant_fits(e, s, id)) cand.push_back({PARENT_METHOD_ATOMIC, id});
+    if (mfma_fits(e, s)) cand.push_back({PARENT_METHOD_MFMA, 0});
     for (auto& mv : cand) {
         const int m = mv.first, id = mv.second;
         int tr, tc;
-        tile_shape(e, m, id, &tr, &tc);
+        tile_shape(s, m, id, &tr, &tc);
         const int block = m == PARENT_METHOD_MFMA ? 256 : atomic_config(id).block;
         std::vector<int> t = tune_work(D, tr, tc, rows);
-        const double updates = (double)tile_pairs(D, t, tr, tc) * e->N;
+        const double updates = (double)tile_pairs(D, t, tr, tc) * tj.N;
         HIPCHK(e, dt.ensure(std::max<size_t>(t.size(), 2) * sizeof(int)));
         HIPCHK(e, hipMemcpyAsync(dt.p, t.data(), t.size() * sizeof(int), hipMemcpyHostToDevice, e->stream));
         double ms = 0;
-        const int nt = (int)(t.size() / 2);
-        if (int rc = time_launch(e, [&] { return launch_variant(e, in, m, id, dt.as<int>(), nt, nullptr, nullptr, nullptr); }, &ms))
+        const TileQueue q{dt.as<int>(), (int)(t.size() / 2), 2, nullptr};
+        if (int rc = time_launch(e, [&] { return launch_tiled(e, in, bmax2d(s), m, id, q, nullptr, nullptr, nullptr); }, &ms))
             return rc;
         parent_tune_entry te{m, id, tr, tc, block, ms, updates, updates / (ms * 1e-3)};
         e->last_tune.push_back(te);
@@ -290,20 +438,19 @@ int run_autotune(parent_engine* e) {
             best_v = id;
         }
     }
-    dt.release();
-    e->tuned.push_back({e->bins[3], e->bins[4], e->bins[5], e->N, D, best_m, best_v});
+    e->tuned.push_back({s.bins[3], s.bins[4], s.bins[5], tj.N, D, best_m, best_v});
     return PARENT_OK;
 }
 
-// The tuned (method, variant) for the current problem, tuning on first use.
-int tuned_choice(parent_engine* e, int* method, int* variant) {
+// The tuned (method, variant) for this problem, tuning on first use.
+int tuned_choice(parent_engine* e, const Traj& tj, BinSet& s, int* method, int* variant) {
     for (auto& t : e->tuned)
-        if (t.b == e->bins[3] && t.a == e->bins[4] && t.d == e->bins[5] && t.N == e->N && t.D == e->D) {
+        if (t.b == s.bins[3] && t.a == s.bins[4] && t.d == s.bins[5] && t.N == tj.N && t.D == tj.D) {
             *method = t.method;
             *variant = t.variant;
             return PARENT_OK;
         }
-    if (int rc = run_autotune(e)) return rc;
+    if (int rc = run_autotune(e, tj, s)) return rc;
     const auto& t = e->tuned.back();
     *method = t.method;
     *variant = t.variant;
@@ -313,16 +460,17 @@ int tuned_choice(parent_engine* e, int* method, int* variant) {
 // The kernel a request runs on: AUTO takes the autotuner's choice, MFMA falls
 // back to the atomic kernels outside its bin range, ATOMIC falls back to the
 // plain kernel when no tile configuration fits the LDS.
-int resolve_method(parent_engine* e, int method, int* used, int* variant) {
+int resolve_method(parent_engine* e, const Traj& tj, BinSet& s, int method, int* used, int* variant) {
+    if (method < PARENT_METHOD_PLAIN || method > PARENT_METHOD_AUTO) return fail(e, PARENT_E_INVALID, "unknown method");
     *used = method;
     *variant = 0;
-    if (method == PARENT_METHOD_AUTO) return tuned_choice(e, used, variant);
+    if (method == PARENT_METHOD_AUTO) return tuned_choice(e, tj, s, used, variant);
     if (method == PARENT_METHOD_MFMA) {
-        if (mfma_fits(e)) return PARENT_OK;
+        if (mfma_fits(e, s)) return PARENT_OK;
         *used = PARENT_METHOD_ATOMIC;
     }
     if (*used == PARENT_METHOD_ATOMIC) {
-        *variant = choose_atomic_config(e);
+        *variant = choose_atomic_config(e, s);
         if (*variant < 0) {
             *used = PARENT_METHOD_PLAIN;
             *variant = 0;
@@ -330,6 +478,142 @@ int resolve_method(parent_engine* e, int method, int* used, int* variant) {
     }
     return PARENT_OK;
 }
+
+// Chooses set s's kernel and uploads its tile list (synchronises; done
+// before any capture).
+int plan_pairs(parent_engine* e, const Traj& t, BinSet& s, int method) {
+    const int bmax = bmax2d(s);
+    if ((size_t)bmax * bmax * 4 > (size_t)e->lds_max) return fail(e, PARENT_E_UNSUPPORTED, "2D histogram does not fit in LDS");
+    int used = method, variant = 0;
+    if (int rc = resolve_method(e, t, s, method, &used, &variant)) return rc;
+    if (used != PARENT_METHOD_PLAIN)
+        if (int rc = ensure_tiles(e, t, s, used, variant)) return rc;
+    s.method = used;
+    s.variant = variant;
+    return PARENT_OK;
+}
+
+// All pairs of set s with its planned kernel: kernels only (capturable).
+hipError_t enqueue_pairs(parent_engine* e, const Traj& t, BinSet& s) {
+    const PairInputs in = with_nlogn(e, pair_inputs(t, s));
+    if (s.method == PARENT_METHOD_PLAIN) return launch_pairs_plain(in, nullptr, e->stream);
+    const TileQueue q{s.tiles.dev.as<int>(), s.tiles.ntiles, 2, nullptr};
+    return launch_tiled(e, in, bmax2d(s), s.method, s.variant, q, nullptr, nullptr, nullptr);
+}
+
+int run_pairs(parent_engine* e, const Traj& t, BinSet& s, int method) {
+    if (!s.prepared) return fail(e, PARENT_E_INVALID, "parent_engine_prepare has not succeeded");
+    if (int rc = plan_pairs(e, t, s, method)) return rc;
+    HIPCHK(e, hipEventRecord(e->ev[4], e->stream));
+    HIPCHK(e, enqueue_pairs(e, t, s));
+    HIPCHK(e, hipEventRecord(e->ev[5], e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    e->stats.ms_pairs = elapsed(e->ev[4], e->ev[5]);
+    e->stats.pair_updates = (double)n_pairs(t.nD) * t.N;
+    e->stats.method = s.method;
+    e->stats.kernel_variant = s.variant;
+    s.s2d_valid = true;
+    return PARENT_OK;
+}
+
+// MIST over set s's S1D/S2D (synchronises).
+int run_mist(parent_engine* e, BinSet& s, int32_t* edges, double* mi, int32_t* left, double* totals) {
+    if (!s.s2d_valid || s.ent_nD < 1) return fail(e, PARENT_E_INVALID, "no 2D entropies on the device");
+    const int nD = s.ent_nD, V = n_dofs(nD);
+    Scratch& x = e->scratch;
+    HIPCHK(e, x.mkey.ensure(V * 8));
+    HIPCHK(e, x.mupar.ensure(V * 4));
+    HIPCHK(e, x.minpos.ensure(V * 4));
+    HIPCHK(e, x.mstate.ensure(V * 4));
+    HIPCHK(e, x.medges.ensure((size_t)V * 16));
+    HIPCHK(e, x.mmi.ensure(V * 8));
+    HIPCHK(e, x.mleft.ensure(V * 4));
+    HIPCHK(e, x.mflags.ensure(64));
+    MistArgs a{};
+    a.nD = nD;
+    a.V = V;
+    a.s1d = s.s1d.as<double>();
+    a.s2d = s.s2d.as<double>();
+    a.key = x.mkey.as<double>();
+    a.upar = x.mupar.as<int>();
+    a.inpos = x.minpos.as<int>();
+    a.state = x.mstate.as<int>();
+    a.edges = x.medges.as<int>();
+    a.mi = x.mmi.as<double>();
+    a.left = x.mleft.as<int>();
+    a.flags = x.mflags.as<int>();
+    int flags = 0;
+    HIPCHK(e, hipMemcpyAsync(x.mflags.p, &flags, sizeof(int), hipMemcpyHostToDevice, e->stream));
+    HIPCHK(e, hipEventRecord(e->ev[6], e->stream));
+    HIPCHK(e, launch_mist(a, e->stream));
+    HIPCHK(e, hipEventRecord(e->ev[7], e->stream));
+    std::vector<double> s1(V);
+    HIPCHK(e, hipMemcpyAsync(edges, a.edges, (size_t)(V - 1) * 16, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(e, hipMemcpyAsync(mi, a.mi, (size_t)(V - 1) * 8, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(e, hipMemcpyAsync(left, a.left, (size_t)(V - 1) * 4, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(e, hipMemcpyAsync(s1.data(), a.s1d, (size_t)V * 8, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(e, hipMemcpyAsync(&flags, x.mflags.p, sizeof(int), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    e->stats.ms_mist = elapsed(e->ev[6], e->ev[7]);
+    if (flags & kFlagMist) return fail(e, PARENT_E_DATA, "MIST: no finite mutual information left to pick");
+    // totals in the reference's summation order (get_PAR_MIST.cpp:276-284, 663-741, 754-765)
+    double t1[3] = {0, 0, 0};
+    for (int t = 0; t < 3; t++)
+        for (int i = 0; i < count_of_type(nD, t); i++) t1[t] += s1[first_of_type(nD, t) + i];
+    double cat[6] = {0, 0, 0, 0, 0, 0}, total_mi = 0;
+    for (int k = 0; k < V - 1; k++) {
+        const int ta = edges[4 * k], tb = edges[4 * k + 1];
+        const int lo = std::min(ta, tb), hi = std::max(ta, tb);
+        const int c = lo == 0 ? hi : (lo == 1 ? 2 + hi : 5);
+        total_mi += mi[k];
+        cat[c] += mi[k];
+    }
+    totals[PARENT_TOT_S1D_B] = t1[0];
+    totals[PARENT_TOT_S1D_A] = t1[1];
+    totals[PARENT_TOT_S1D_D] = t1[2];
+    for (int c = 0; c < 6; c++) totals[PARENT_TOT_MI_BB + c] = cat[c];
+    totals[PARENT_TOT_TOTAL] = t1[0] + t1[1] + t1[2] - total_mi;
+    return PARENT_OK;
+}
+
+int read_s1d(parent_engine* e, BinSet& s, double* out) {
+    if (s.ent_nD < 1) return fail(e, PARENT_E_INVALID, "no entropies computed");
+    HIPCHK(e, hipMemcpyAsync(out, s.s1d.p, (size_t)n_dofs(s.ent_nD) * 8, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    return PARENT_OK;
+}
+
+int read_s2d(parent_engine* e, BinSet& s, double* out) {
+    if (!s.s2d_valid) return fail(e, PARENT_E_INVALID, "parent_engine_pairs has not succeeded");
+    HIPCHK(e, hipMemcpyAsync(out, s.s2d.p, (size_t)n_pairs(s.ent_nD) * 8, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    return PARENT_OK;
+}
+
+void drop_graph(parent_engine* e) {
+    if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    e->graph_exec = nullptr;
+    e->graph = nullptr;
+    e->graph_key.clear();
+}
+
+int create_stream_and_events(parent_engine* e, int device) {
+    hipError_t h;
+    if ((h = hipSetDevice(device)) != hipSuccess || (h = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking)) != hipSuccess) {
+        set_last_error(std::string("HIP init failed: ") + hipGetErrorString(h));
+        return PARENT_E_DEVICE;
+    }
+    for (auto& ev : e->ev) {
+        if ((h = hipEventCreate(&ev)) != hipSuccess) {
+            set_last_error(std::string("hipEventCreate failed: ") + hipGetErrorString(h));
+            return PARENT_E_DEVICE;
+        }
+    }
+    return PARENT_OK;
+}
+
+int bfail(parent_batch* b, int code, const std::string& msg) { return fail(b->eng, code, msg); }
 
 }  // namespace
 
@@ -350,17 +634,10 @@ int parent_engine_create(int device, parent_engine** out) {
     }
     parent_engine* e = new parent_engine();
     e->device = device;
-    if ((h = hipSetDevice(device)) != hipSuccess || (h = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking)) != hipSuccess) {
-        set_last_error(std::string("HIP init failed: ") + hipGetErrorString(h));
-        delete e;
-        return PARENT_E_DEVICE;
-    }
-    for (auto& ev : e->ev) {
-        if ((h = hipEventCreate(&ev)) != hipSuccess) {
-            set_last_error(std::string("hipEventCreate failed: ") + hipGetErrorString(h));
-            parent_engine_destroy(e);
-            return PARENT_E_DEVICE;
-        }
+    e->sets.emplace_back(new BinSet());
+    if (int rc = create_stream_and_events(e, device)) {
+        parent_engine_destroy(e);
+        return rc;
     }
     int lds = 0;
     if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
@@ -373,11 +650,10 @@ void parent_engine_destroy(parent_engine* e) {
     if (!e) return;
     (void)hipSetDevice(e->device);
     if (e->stream) (void)hipStreamSynchronize(e->stream);
-    DevBuf* bufs[] = {&e->values_own, &e->emin, &e->emax, &e->mfhist, &e->flags, &e->vmin, &e->vmax,
-                      &e->modfit, &e->bins2, &e->bins1, &e->s1d, &e->c2, &e->jtab, &e->nlogn, &e->s2d,
-                      &e->mkey, &e->mupar, &e->minpos, &e->mstate, &e->medges, &e->mmi, &e->mleft,
-                      &e->tiles.dev};
-    for (DevBuf* b : bufs) b->release();
+    drop_graph(e);
+    e->sets.clear();
+    e->traj.values_own.release();
+    e->scratch = Scratch();
     for (auto& ev : e->ev)
         if (ev) (void)hipEventDestroy(ev);
     if (e->stream) (void)hipStreamDestroy(e->stream);
@@ -392,160 +668,159 @@ int parent_engine_set_trajectory(parent_engine* e, int n_dihedrals, int n_frames
     if (n_dihedrals < 1 || n_frames < 1 || (double_prec != 0 && double_prec != 1))
         return fail(e, PARENT_E_INVALID, "need n_dihedrals >= 1, n_frames >= 1, double_prec in {0,1}");
     HIPCHK(e, hipSetDevice(e->device));
-    e->nD = n_dihedrals;
-    e->N = n_frames;
-    e->dp = double_prec;
-    e->D = n_dofs(n_dihedrals);
-    e->Np = (int)(((long long)n_frames + kFrameAlign - 1) / kFrameAlign * kFrameAlign);
-    e->prepared = false;
-    e->s2d_valid = false;
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    drop_graph(e);
+    e->traj_serial++;
+    Traj& t = e->traj;
+    t.nD = n_dihedrals;
+    t.N = n_frames;
+    t.dp = double_prec;
+    t.D = n_dofs(n_dihedrals);
+    t.Np = (int)(((long long)n_frames + kFrameAlign - 1) / kFrameAlign * kFrameAlign);
+    t.values = nullptr;
+    e->sets.resize(1);
+    e->active = 0;
+    e->sets[0]->prepared = false;
+    e->sets[0]->s2d_valid = false;
+    e->sets[0]->ent_nD = -1;
     e->stats = parent_stats{};
-    const size_t bytes = (size_t)e->N * e->D * (double_prec ? 8 : 4);
+    const size_t bytes = (size_t)t.N * t.D * (double_prec ? 8 : 4);
     if (on_device) {
-        e->values = values;
+        t.values_own.release();
+        t.values = values;
     } else {
-        HIPCHK(e, e->values_own.ensure(bytes));
+        HIPCHK(e, t.values_own.ensure(bytes));
         HIPCHK(e, hipEventRecord(e->ev[0], e->stream));
-        HIPCHK(e, hipMemcpyAsync(e->values_own.p, values, bytes, hipMemcpyHostToDevice, e->stream));
+        HIPCHK(e, hipMemcpyAsync(t.values_own.p, values, bytes, hipMemcpyHostToDevice, e->stream));
         HIPCHK(e, hipEventRecord(e->ev[1], e->stream));
         HIPCHK(e, hipStreamSynchronize(e->stream));
         e->stats.ms_upload = elapsed(e->ev[0], e->ev[1]);
-        e->values = e->values_own.p;
+        t.values = t.values_own.p;
     }
     return PARENT_OK;
 }
 
 int parent_engine_prepare(parent_engine* e, const int bins[6]) {
     if (!e || !bins) return PARENT_E_INVALID;
-    if (!e->values) return fail(e, PARENT_E_INVALID, "no trajectory set");
-    for (int k = 0; k < 6; k++) {
-        if (bins[k] < 1) return fail(e, PARENT_E_INVALID, "number of bins < 1");
-        if (bins[k] > kMaxBins) return fail(e, PARENT_E_UNSUPPORTED, "more than 255 bins per dof is not supported");
-    }
+    if (!e->traj.values) return fail(e, PARENT_E_INVALID, "no trajectory set");
+    if (int rc = check_bins(e, bins)) return rc;
     HIPCHK(e, hipSetDevice(e->device));
-    memcpy(e->bins, bins, sizeof(e->bins));
-    e->prepared = false;
-    e->s2d_valid = false;
-    const int D = e->D, nD = e->nD, N = e->N;
-    const bool need1 = bins[0] != bins[3] || bins[1] != bins[4] || bins[2] != bins[5];
-    HIPCHK(e, e->emin.ensure(D * 8));
-    HIPCHK(e, e->emax.ensure(D * 8));
-    HIPCHK(e, e->mfhist.ensure((size_t)nD * kModfitBins * 4));
-    HIPCHK(e, e->flags.ensure(64));
-    HIPCHK(e, e->vmin.ensure(D * 8));
-    HIPCHK(e, e->vmax.ensure(D * 8));
-    HIPCHK(e, e->modfit.ensure(D * 8));
-    HIPCHK(e, e->bins2.ensure((size_t)D * e->Np));
-    if (need1) HIPCHK(e, e->bins1.ensure((size_t)D * e->Np));
-    HIPCHK(e, e->s1d.ensure(D * 8));
-    HIPCHK(e, e->c2.ensure(D * 8));
-    HIPCHK(e, e->jtab.ensure((size_t)D * 256 * 8));
-    HIPCHK(e, e->nlogn.ensure(((size_t)N + 1) * 8));
-    PrepareArgs a{};
-    a.values = e->values;
-    a.double_prec = e->dp;
-    a.nD = nD;
-    a.D = D;
-    a.N = N;
-    a.Np = e->Np;
-    a.pi = acos(-1.0);  // PARENT.cpp:112
-    a.B1 = bins1d(e);
-    a.B2 = bins2d(e);
-    a.emin = e->emin.as<unsigned long long>();
-    a.emax = e->emax.as<unsigned long long>();
-    a.mfhist = e->mfhist.as<unsigned>();
-    a.flags = e->flags.as<int>();
-    a.vmin = e->vmin.as<double>();
-    a.vmax = e->vmax.as<double>();
-    a.modfit = e->modfit.as<double>();
-    a.bins2 = e->bins2.as<uint8_t>();
-    a.bins1 = need1 ? e->bins1.as<uint8_t>() : nullptr;
-    a.s1d = e->s1d.as<double>();
-    a.c2 = e->c2.as<double>();
-    a.jtab = e->jtab.as<double>();
-    a.nlogn = e->nlogn.as<double>();
-    HIPCHK(e, hipEventRecord(e->ev[2], e->stream));
-    HIPCHK(e, launch_prepare(a, e->stream));
-    HIPCHK(e, hipEventRecord(e->ev[3], e->stream));
-    int flags = 0;
-    HIPCHK(e, hipMemcpyAsync(&flags, e->flags.p, sizeof(int), hipMemcpyDeviceToHost, e->stream));
-    HIPCHK(e, hipStreamSynchronize(e->stream));
-    e->stats.ms_prepare = elapsed(e->ev[2], e->ev[3]);
-    if (flags & kFlagNegBond) return fail(e, PARENT_E_DATA, "ERROR: a bond length is smaller than 0.0");
-    if (flags & kFlagRange)
-        return fail(e, PARENT_E_RANGE, "ERROR: a value falls outside its histogram range (angle >= pi or dihedral outside [0, 2pi+5e-9))");
-    e->prepared = true;
-    e->ent_nD = nD;
-    return PARENT_OK;
+    drop_graph(e);  // the captured sweep refers to the sets' buffers
+    e->sets.resize(1);
+    e->active = 0;
+    BinSet& s = *e->sets[0];
+    memcpy(s.bins, bins, sizeof(s.bins));
+    return run_prepare(e, e->traj, s);
 }
 
 int parent_engine_pairs(parent_engine* e, int method) {
     if (!e) return PARENT_E_INVALID;
-    if (int rc = check_ready(e)) return rc;
     HIPCHK(e, hipSetDevice(e->device));
-    const long long P = n_pairs(e->nD);
-    HIPCHK(e, e->s2d.ensure((size_t)std::max<long long>(P, 1) * 8));
-    const int bmax = bmax2d(e);
-    if ((size_t)bmax * bmax * 4 > (size_t)e->lds_max)
-        return fail(e, PARENT_E_UNSUPPORTED, "2D histogram does not fit in LDS");
-    PairInputs in = pair_inputs(e);
-    int used = method, variant = 0;
-    if (int rc = resolve_method(e, method, &used, &variant)) return rc;
-    if (used != PARENT_METHOD_PLAIN) {
-        if (int rc = ensure_tiles(e, used, variant)) return rc;
+    drop_graph(e);  // may re-plan a set the captured sweep refers to
+    return run_pairs(e, e->traj, cur(e), method);
+}
+
+int parent_engine_sweep(parent_engine* e, int n_sets, const int* bins, int method) {
+    if (!e || n_sets < 1 || !bins) return PARENT_E_INVALID;
+    if (!e->traj.values) return fail(e, PARENT_E_INVALID, "no trajectory set");
+    for (int k = 0; k < n_sets; k++)
+        if (int rc = check_bins(e, bins + 6 * k)) return rc;
+    HIPCHK(e, hipSetDevice(e->device));
+    const Traj& t = e->traj;
+    std::vector<int> key(bins, bins + 6 * n_sets);
+    key.push_back(method);
+    key.push_back((int)e->traj_serial);
+    key.push_back(e->forced_atomic);
+    if (key != e->graph_key) drop_graph(e);
+    while ((int)e->sets.size() < n_sets) e->sets.emplace_back(new BinSet());
+    e->sets.resize(n_sets);
+    e->active = 0;
+    // eager pass: buffers, data checks, kernel choice (autotuning) and tile
+    // lists, all of which synchronise and so stay outside the capture
+    if (!e->graph_exec) {
+        for (int k = 0; k < n_sets; k++) {
+            BinSet& s = *e->sets[k];
+            memcpy(s.bins, bins + 6 * k, sizeof(s.bins));
+            if (int rc = run_prepare(e, t, s)) return rc;
+            if (int rc = plan_pairs(e, t, s, method)) return rc;
+        }
+        HIPCHK(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeRelaxed));
+        hipError_t h = hipSuccess;
+        for (int k = 0; k < n_sets && h == hipSuccess; k++) {
+            h = enqueue_prepare(e, t, *e->sets[k]);
+            if (h == hipSuccess) h = enqueue_pairs(e, t, *e->sets[k]);
+        }
+        hipGraph_t g = nullptr;
+        const hipError_t he = hipStreamEndCapture(e->stream, &g);
+        if (h != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return hip_fail(e, h, "sweep capture");
+        }
+        HIPCHK(e, he);
+        e->graph = g;
+        HIPCHK(e, hipGraphInstantiate(&e->graph_exec, e->graph, nullptr, nullptr, 0));
+        e->graph_key = key;
     }
     HIPCHK(e, hipEventRecord(e->ev[4], e->stream));
-    if (used == PARENT_METHOD_PLAIN) {
-        HIPCHK(e, launch_pairs_plain(in, nullptr, e->stream));
-    } else if (used == PARENT_METHOD_ATOMIC || used == PARENT_METHOD_MFMA) {
-        HIPCHK(e, launch_variant(e, in, used, variant, e->tiles.dev.as<int>(), e->tiles.ntiles, nullptr, nullptr, nullptr));
-    } else {
-        return fail(e, PARENT_E_INVALID, "unknown method");
-    }
+    HIPCHK(e, hipGraphLaunch(e->graph_exec, e->stream));
     HIPCHK(e, hipEventRecord(e->ev[5], e->stream));
     HIPCHK(e, hipStreamSynchronize(e->stream));
-    e->stats.ms_pairs = elapsed(e->ev[4], e->ev[5]);
-    e->stats.pair_updates = (double)P * e->N;
-    e->stats.method = used;
-    e->stats.kernel_variant = variant;
-    e->s2d_valid = true;
+    double upd = 0;
+    for (int k = 0; k < n_sets; k++) {
+        BinSet& s = *e->sets[k];
+        if (int rc = check_flags(e, s)) return rc;
+        s.prepared = true;
+        s.s2d_valid = true;
+        s.ent_nD = t.nD;
+        upd += (double)n_pairs(t.nD) * t.N;
+    }
+    e->stats.ms_sweep = elapsed(e->ev[4], e->ev[5]);
+    e->stats.sweep_sets = n_sets;
+    e->stats.pair_updates = upd;
+    return PARENT_OK;
+}
+
+int parent_engine_select(parent_engine* e, int set) {
+    if (!e) return PARENT_E_INVALID;
+    if (set < 0 || set >= (int)e->sets.size()) return fail(e, PARENT_E_INVALID, "bin set index out of range");
+    e->active = set;
+    e->stats.method = e->sets[set]->method;
+    e->stats.kernel_variant = e->sets[set]->variant;
     return PARENT_OK;
 }
 
 int parent_engine_read_s1d(parent_engine* e, double* out) {
     if (!e || !out) return PARENT_E_INVALID;
-    if (e->ent_nD < 0) return fail(e, PARENT_E_INVALID, "no entropies computed");
     HIPCHK(e, hipSetDevice(e->device));
-    HIPCHK(e, hipMemcpyAsync(out, e->s1d.p, (size_t)n_dofs(e->ent_nD) * 8, hipMemcpyDeviceToHost, e->stream));
-    HIPCHK(e, hipStreamSynchronize(e->stream));
-    return PARENT_OK;
+    return read_s1d(e, cur(e), out);
 }
 
 int parent_engine_read_s2d(parent_engine* e, double* out) {
     if (!e || !out) return PARENT_E_INVALID;
-    if (!e->s2d_valid) return fail(e, PARENT_E_INVALID, "parent_engine_pairs has not succeeded");
     HIPCHK(e, hipSetDevice(e->device));
-    HIPCHK(e, hipMemcpyAsync(out, e->s2d.p, (size_t)n_pairs(e->ent_nD) * 8, hipMemcpyDeviceToHost, e->stream));
-    HIPCHK(e, hipStreamSynchronize(e->stream));
-    return PARENT_OK;
+    return read_s2d(e, cur(e), out);
 }
 
 int parent_engine_read_ranges(parent_engine* e, double* vmin, double* vmax, double* modfit) {
     if (!e) return PARENT_E_INVALID;
-    if (int rc = check_ready(e)) return rc;
+    BinSet& s = cur(e);
+    if (!s.prepared) return fail(e, PARENT_E_INVALID, "parent_engine_prepare has not succeeded");
     HIPCHK(e, hipSetDevice(e->device));
-    if (vmin) HIPCHK(e, hipMemcpyAsync(vmin, e->vmin.p, e->D * 8, hipMemcpyDeviceToHost, e->stream));
-    if (vmax) HIPCHK(e, hipMemcpyAsync(vmax, e->vmax.p, e->D * 8, hipMemcpyDeviceToHost, e->stream));
-    if (modfit) HIPCHK(e, hipMemcpyAsync(modfit, e->modfit.p, e->D * 8, hipMemcpyDeviceToHost, e->stream));
+    const size_t D = e->traj.D;
+    if (vmin) HIPCHK(e, hipMemcpyAsync(vmin, s.vmin.p, D * 8, hipMemcpyDeviceToHost, e->stream));
+    if (vmax) HIPCHK(e, hipMemcpyAsync(vmax, s.vmax.p, D * 8, hipMemcpyDeviceToHost, e->stream));
+    if (modfit) HIPCHK(e, hipMemcpyAsync(modfit, s.modfit.p, D * 8, hipMemcpyDeviceToHost, e->stream));
     HIPCHK(e, hipStreamSynchronize(e->stream));
     return PARENT_OK;
 }
 
 int parent_engine_read_bins(parent_engine* e, uint8_t* out) {
     if (!e || !out) return PARENT_E_INVALID;
-    if (int rc = check_ready(e)) return rc;
+    BinSet& s = cur(e);
+    if (!s.prepared) return fail(e, PARENT_E_INVALID, "parent_engine_prepare has not succeeded");
     HIPCHK(e, hipSetDevice(e->device));
-    HIPCHK(e, hipMemcpy2DAsync(out, e->N, e->bins2.p, e->Np, e->N, e->D, hipMemcpyDeviceToHost, e->stream));
+    const Traj& t = e->traj;
+    HIPCHK(e, hipMemcpy2DAsync(out, t.N, s.bins2.p, t.Np, t.N, t.D, hipMemcpyDeviceToHost, e->stream));
     HIPCHK(e, hipStreamSynchronize(e->stream));
     return PARENT_OK;
 }
@@ -553,24 +828,25 @@ int parent_engine_read_bins(parent_engine* e, uint8_t* out) {
 int parent_engine_pair_counts(parent_engine* e, int method, int n_pairs_req, const int32_t* pairs,
                               int32_t* counts) {
     if (!e || n_pairs_req < 0 || (n_pairs_req > 0 && (!pairs || !counts))) return PARENT_E_INVALID;
-    if (int rc = check_ready(e)) return rc;
+    BinSet& s = cur(e);
+    const Traj& tj = e->traj;
+    if (!s.prepared) return fail(e, PARENT_E_INVALID, "parent_engine_prepare has not succeeded");
     if (n_pairs_req == 0) return PARENT_OK;
     HIPCHK(e, hipSetDevice(e->device));
-    const Bins3 B = bins2d(e);
+    const Bins3 B = bins2d(s);
     std::vector<long long> off(n_pairs_req);
     long long total = 0;
     for (int k = 0; k < n_pairs_req; k++) {
         const int gi = pairs[2 * k], gj = pairs[2 * k + 1];
-        if (gi < 0 || gj >= e->D || gi >= gj) return fail(e, PARENT_E_INVALID, "pairs must satisfy 0 <= gi < gj < D");
+        if (gi < 0 || gj >= tj.D || gi >= gj) return fail(e, PARENT_E_INVALID, "pairs must satisfy 0 <= gi < gj < D");
         off[k] = total;
-        total += (long long)B.of(dof_type(e->nD, gi)) * B.of(dof_type(e->nD, gj));
+        total += (long long)B.of(dof_type(tj.nD, gi)) * B.of(dof_type(tj.nD, gj));
     }
     DevBuf dpairs, doff, dout;
     HIPCHK(e, dout.ensure(total * 4));
-    PairInputs in = pair_inputs(e);
+    const PairInputs in = with_nlogn(e, pair_inputs(tj, s));
     int used = method, variant = 0;
-    if (int rc = resolve_method(e, method, &used, &variant)) return rc;
-    int rc = PARENT_OK;
+    if (int rc = resolve_method(e, tj, s, method, &used, &variant)) return rc;
     if (used == PARENT_METHOD_PLAIN) {
         HIPCHK(e, dpairs.ensure((size_t)n_pairs_req * 8));
         HIPCHK(e, doff.ensure((size_t)n_pairs_req * 8));
@@ -578,22 +854,21 @@ int parent_engine_pair_counts(parent_engine* e, int method, int n_pairs_req, con
         HIPCHK(e, hipMemcpyAsync(doff.p, off.data(), (size_t)n_pairs_req * 8, hipMemcpyHostToDevice, e->stream));
         CountsDump dump{n_pairs_req, dpairs.as<int>(), doff.as<long long>(), dout.as<int>()};
         HIPCHK(e, launch_pairs_plain(in, &dump, e->stream));
-    } else if (used == PARENT_METHOD_ATOMIC || used == PARENT_METHOD_MFMA) {
+    } else {
         // the tiles holding the requested pairs, each pair as (slot, offset)
-        AtomicConfig c{};
-        tile_shape(e, used, variant, &c.tr, &c.tc);
-        const int stride = 2;
+        int tr, tc;
+        tile_shape(s, used, variant, &tr, &tc);
         std::vector<std::vector<int>> per_tile;
         std::vector<int> tl;
         std::map<long long, int> key;  // tile key -> index
         for (int k = 0; k < n_pairs_req; k++) {
             const int gi = pairs[2 * k], gj = pairs[2 * k + 1];
-            const int r0 = gi / c.tr * c.tr, c0 = gj / c.tc * c.tc;
-            const long long tk = (long long)r0 * e->D + c0;
+            const int r0 = gi / tr * tr, c0 = gj / tc * tc;
+            const long long tk = (long long)r0 * tj.D + c0;
             auto it = key.find(tk);
             int idx;
             if (it == key.end()) {
-                idx = (int)tl.size() / stride;
+                idx = (int)tl.size() / 2;
                 key[tk] = idx;
                 tl.push_back(r0);
                 tl.push_back(c0);
@@ -607,8 +882,8 @@ int parent_engine_pair_counts(parent_engine* e, int method, int n_pairs_req, con
         for (size_t t = 0; t < per_tile.size(); t++) {
             for (int k : per_tile[t]) {
                 const int gi = pairs[2 * k], gj = pairs[2 * k + 1];
-                const int r0 = tl[stride * t], c0 = tl[stride * t + 1];
-                entries.push_back((gi - r0) * c.tc + (gj - c0));
+                const int r0 = tl[2 * t], c0 = tl[2 * t + 1];
+                entries.push_back((gi - r0) * tc + (gj - c0));
                 entries.push_back((int)(off[k] >> 31));
                 entries.push_back((int)(off[k] & 0x7FFFFFFF));
             }
@@ -621,109 +896,49 @@ int parent_engine_pair_counts(parent_engine* e, int method, int n_pairs_req, con
         HIPCHK(e, hipMemcpyAsync(dtiles.p, tl.data(), tl.size() * 4, hipMemcpyHostToDevice, e->stream));
         HIPCHK(e, hipMemcpyAsync(dbegin.p, begin.data(), begin.size() * 4, hipMemcpyHostToDevice, e->stream));
         HIPCHK(e, hipMemcpyAsync(dent.p, entries.data(), entries.size() * 4, hipMemcpyHostToDevice, e->stream));
-        HIPCHK(e, launch_variant(e, in, used, variant, dtiles.as<int>(), (int)(tl.size() / stride), dbegin.as<int>(), dent.as<int>(), dout.as<int>()));
-        HIPCHK(e, hipStreamSynchronize(e->stream));
-        dtiles.release();
-        dbegin.release();
-        dent.release();
-    } else {
-        rc = fail(e, PARENT_E_INVALID, "unknown method");
-    }
-    if (rc == PARENT_OK) {
-        HIPCHK(e, hipMemcpyAsync(counts, dout.p, total * 4, hipMemcpyDeviceToHost, e->stream));
+        const TileQueue q{dtiles.as<int>(), (int)(tl.size() / 2), 2, nullptr};
+        HIPCHK(e, launch_tiled(e, in, bmax2d(s), used, variant, q, dbegin.as<int>(), dent.as<int>(), dout.as<int>()));
         HIPCHK(e, hipStreamSynchronize(e->stream));
     }
-    dpairs.release();
-    doff.release();
-    dout.release();
-    return rc;
+    HIPCHK(e, hipMemcpyAsync(counts, dout.p, total * 4, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    return PARENT_OK;
 }
 
 int parent_engine_set_entropies(parent_engine* e, int n_dihedrals, const double* s1d, const double* s2d) {
     if (!e || !s1d || !s2d || n_dihedrals < 1) return PARENT_E_INVALID;
     HIPCHK(e, hipSetDevice(e->device));
+    drop_graph(e);
+    BinSet& s = cur(e);
     const size_t D = n_dofs(n_dihedrals), P = n_pairs(n_dihedrals);
-    HIPCHK(e, e->s1d.ensure(D * 8));
-    HIPCHK(e, e->s2d.ensure(P * 8));
-    HIPCHK(e, hipMemcpyAsync(e->s1d.p, s1d, D * 8, hipMemcpyHostToDevice, e->stream));
-    HIPCHK(e, hipMemcpyAsync(e->s2d.p, s2d, P * 8, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(e, s.s1d.ensure(D * 8));
+    HIPCHK(e, s.s2d.ensure(std::max<size_t>(P, 1) * 8));
+    HIPCHK(e, hipMemcpyAsync(s.s1d.p, s1d, D * 8, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(e, hipMemcpyAsync(s.s2d.p, s2d, P * 8, hipMemcpyHostToDevice, e->stream));
     HIPCHK(e, hipStreamSynchronize(e->stream));
-    e->ent_nD = n_dihedrals;
-    e->s2d_valid = true;
-    e->prepared = false;
+    s.ent_nD = n_dihedrals;
+    s.s2d_valid = true;
+    s.prepared = false;
     return PARENT_OK;
 }
 
 int parent_engine_mist(parent_engine* e, int32_t* edges, double* mi, int32_t* left, double* totals) {
     if (!e || !edges || !mi || !left || !totals) return PARENT_E_INVALID;
-    if (!e->s2d_valid) return fail(e, PARENT_E_INVALID, "no 2D entropies on the device");
     HIPCHK(e, hipSetDevice(e->device));
-    const int nD = e->ent_nD, V = n_dofs(nD);
-    HIPCHK(e, e->mkey.ensure(V * 8));
-    HIPCHK(e, e->mupar.ensure(V * 4));
-    HIPCHK(e, e->minpos.ensure(V * 4));
-    HIPCHK(e, e->mstate.ensure(V * 4));
-    HIPCHK(e, e->medges.ensure((size_t)V * 16));
-    HIPCHK(e, e->mmi.ensure(V * 8));
-    HIPCHK(e, e->mleft.ensure(V * 4));
-    HIPCHK(e, e->flags.ensure(64));
-    MistArgs a{};
-    a.nD = nD;
-    a.V = V;
-    a.s1d = e->s1d.as<double>();
-    a.s2d = e->s2d.as<double>();
-    a.key = e->mkey.as<double>();
-    a.upar = e->mupar.as<int>();
-    a.inpos = e->minpos.as<int>();
-    a.state = e->mstate.as<int>();
-    a.edges = e->medges.as<int>();
-    a.mi = e->mmi.as<double>();
-    a.left = e->mleft.as<int>();
-    a.flags = e->flags.as<int>();
-    int flags = 0;
-    HIPCHK(e, hipMemcpyAsync(e->flags.p, &flags, sizeof(int), hipMemcpyHostToDevice, e->stream));
-    HIPCHK(e, hipEventRecord(e->ev[6], e->stream));
-    HIPCHK(e, launch_mist(a, e->stream));
-    HIPCHK(e, hipEventRecord(e->ev[7], e->stream));
-    std::vector<double> s1(V);
-    HIPCHK(e, hipMemcpyAsync(edges, a.edges, (size_t)(V - 1) * 16, hipMemcpyDeviceToHost, e->stream));
-    HIPCHK(e, hipMemcpyAsync(mi, a.mi, (size_t)(V - 1) * 8, hipMemcpyDeviceToHost, e->stream));
-    HIPCHK(e, hipMemcpyAsync(left, a.left, (size_t)(V - 1) * 4, hipMemcpyDeviceToHost, e->stream));
-    HIPCHK(e, hipMemcpyAsync(s1.data(), a.s1d, (size_t)V * 8, hipMemcpyDeviceToHost, e->stream));
-    HIPCHK(e, hipMemcpyAsync(&flags, e->flags.p, sizeof(int), hipMemcpyDeviceToHost, e->stream));
-    HIPCHK(e, hipStreamSynchronize(e->stream));
-    e->stats.ms_mist = elapsed(e->ev[6], e->ev[7]);
-    if (flags & kFlagMist) return fail(e, PARENT_E_DATA, "MIST: no finite mutual information left to pick");
-    // totals in the reference's summation order (get_PAR_MIST.cpp:276-284, 663-741, 754-765)
-    double t1[3] = {0, 0, 0};
-    for (int t = 0; t < 3; t++)
-        for (int i = 0; i < count_of_type(nD, t); i++) t1[t] += s1[first_of_type(nD, t) + i];
-    double cat[6] = {0, 0, 0, 0, 0, 0}, total_mi = 0;
-    for (int s = 0; s < V - 1; s++) {
-        const int ta = edges[4 * s], tb = edges[4 * s + 1];
-        const int lo = std::min(ta, tb), hi = std::max(ta, tb);
-        const int c = lo == 0 ? hi : (lo == 1 ? 2 + hi : 5);
-        total_mi += mi[s];
-        cat[c] += mi[s];
-    }
-    totals[PARENT_TOT_S1D_B] = t1[0];
-    totals[PARENT_TOT_S1D_A] = t1[1];
-    totals[PARENT_TOT_S1D_D] = t1[2];
-    for (int c = 0; c < 6; c++) totals[PARENT_TOT_MI_BB + c] = cat[c];
-    totals[PARENT_TOT_TOTAL] = t1[0] + t1[1] + t1[2] - total_mi;
-    return PARENT_OK;
+    return run_mist(e, cur(e), edges, mi, left, totals);
 }
 
 int parent_engine_autotune(parent_engine* e, parent_tune_entry* entries, int max_entries, int* n_entries) {
     if (!e) return PARENT_E_INVALID;
-    if (int rc = check_ready(e)) return rc;
+    BinSet& s = cur(e);
+    const Traj& t = e->traj;
+    if (!s.prepared) return fail(e, PARENT_E_INVALID, "parent_engine_prepare has not succeeded");
     HIPCHK(e, hipSetDevice(e->device));
-    const long long P = n_pairs(e->nD);
-    HIPCHK(e, e->s2d.ensure((size_t)std::max<long long>(P, 1) * 8));
-    e->tuned.erase(std::remove_if(e->tuned.begin(), e->tuned.end(), [&](const parent_engine::Tuned& t) {
-        return t.b == e->bins[3] && t.a == e->bins[4] && t.d == e->bins[5] && t.N == e->N && t.D == e->D;
+    e->tuned.erase(std::remove_if(e->tuned.begin(), e->tuned.end(), [&](const parent_engine::Tuned& x) {
+        return x.b == s.bins[3] && x.a == s.bins[4] && x.d == s.bins[5] && x.N == t.N && x.D == t.D;
     }), e->tuned.end());
-    if (int rc = run_autotune(e)) return rc;
+    drop_graph(e);
+    if (int rc = run_autotune(e, t, s)) return rc;
     const int n = (int)e->last_tune.size();
     if (n_entries) *n_entries = n;
     for (int k = 0; k < n && k < max_entries && entries; k++) entries[k] = e->last_tune[k];
@@ -755,6 +970,200 @@ int parent_engine_sync(parent_engine* e) {
 int parent_engine_stats(parent_engine* e, parent_stats* out) {
     if (!e || !out) return PARENT_E_INVALID;
     *out = e->stats;
+    return PARENT_OK;
+}
+
+/* ---------------------------------------------------------------------- */
+/* batch of trajectories                                                   */
+/* ---------------------------------------------------------------------- */
+
+int parent_batch_create(int device, const int bins[6], parent_batch** out) {
+    if (!out || !bins) return PARENT_E_INVALID;
+    *out = nullptr;
+    for (int k = 0; k < 6; k++)
+        if (bins[k] < 1 || bins[k] > kMaxBins) {
+            set_last_error(bins[k] < 1 ? "number of bins < 1" : "more than 255 bins per dof is not supported");
+            return bins[k] < 1 ? PARENT_E_INVALID : PARENT_E_UNSUPPORTED;
+        }
+    parent_engine* e = nullptr;
+    if (int rc = parent_engine_create(device, &e)) return rc;
+    parent_batch* b = new parent_batch();
+    b->eng = e;
+    memcpy(b->bins, bins, sizeof(b->bins));
+    *out = b;
+    return PARENT_OK;
+}
+
+void parent_batch_destroy(parent_batch* b) {
+    if (!b) return;
+    if (b->eng) {
+        (void)hipSetDevice(b->eng->device);
+        (void)hipStreamSynchronize(b->eng->stream);
+    }
+    b->items.clear();
+    b->queue.release();
+    b->descs.release();
+    parent_engine_destroy(b->eng);
+    delete b;
+}
+
+const char* parent_batch_error(const parent_batch* b) { return b && b->eng ? b->eng->err.c_str() : "null batch"; }
+
+int parent_batch_add(parent_batch* b, int n_dihedrals, int n_frames, int double_prec, const void* values,
+                     int on_device, int* index) {
+    if (!b || !values) return PARENT_E_INVALID;
+    parent_engine* e = b->eng;
+    if (n_dihedrals < 1 || n_frames < 1 || (double_prec != 0 && double_prec != 1))
+        return fail(e, PARENT_E_INVALID, "need n_dihedrals >= 1, n_frames >= 1, double_prec in {0,1}");
+    HIPCHK(e, hipSetDevice(e->device));
+    std::unique_ptr<parent_batch::Item> it(new parent_batch::Item());
+    Traj& t = it->traj;
+    t.nD = n_dihedrals;
+    t.N = n_frames;
+    t.dp = double_prec;
+    t.D = n_dofs(n_dihedrals);
+    t.Np = (int)(((long long)n_frames + kFrameAlign - 1) / kFrameAlign * kFrameAlign);
+    if (on_device) {
+        t.values = values;
+    } else {
+        const size_t bytes = (size_t)t.N * t.D * (double_prec ? 8 : 4);
+        HIPCHK(e, t.values_own.ensure(bytes));
+        HIPCHK(e, hipMemcpyAsync(t.values_own.p, values, bytes, hipMemcpyHostToDevice, e->stream));
+        t.values = t.values_own.p;
+    }
+    memcpy(it->set.bins, b->bins, sizeof(b->bins));
+    const int rc = run_prepare(e, t, it->set);
+    // the index matrix and per-dof constants are all the pairs need: the
+    // uploaded values go right away
+    t.values_own.release();
+    t.values = nullptr;
+    if (rc) return rc;
+    if (index) *index = (int)b->items.size();
+    b->items.push_back(std::move(it));
+    b->queue_items = -1;
+    return PARENT_OK;
+}
+
+int parent_batch_run(parent_batch* b, int method) {
+    if (!b) return PARENT_E_INVALID;
+    parent_engine* e = b->eng;
+    if (b->items.empty()) return bfail(b, PARENT_E_INVALID, "empty batch");
+    HIPCHK(e, hipSetDevice(e->device));
+    const int n = (int)b->items.size();
+    // kernel choice on the item with the most pair-frame updates (one tile
+    // shape for the whole queue; the bins are common to all items)
+    int big = 0;
+    for (int k = 1; k < n; k++) {
+        const Traj &a = b->items[k]->traj, &c = b->items[big]->traj;
+        if ((double)n_pairs(a.nD) * a.N > (double)n_pairs(c.nD) * c.N) big = k;
+    }
+    // the n ln n table must cover the longest trajectory
+    int nmax = 0;
+    for (auto& it : b->items) nmax = std::max(nmax, it->traj.N);
+    HIPCHK(e, e->scratch.nlogn.ensure(((size_t)nmax + 1) * 8));
+    HIPCHK(e, launch_nlogn(nmax, e->scratch.nlogn.as<double>(), e->stream));
+    BinSet& sb = b->items[big]->set;
+    const Traj& tb = b->items[big]->traj;
+    int used = method, variant = 0;
+    if (int rc = resolve_method(e, tb, sb, method, &used, &variant)) return rc;
+    if (used == PARENT_METHOD_PLAIN) {
+        // the plain kernel has no tile queue: the tiled atomic kernel runs instead
+        used = PARENT_METHOD_ATOMIC;
+        variant = choose_atomic_config(e, sb);
+        if (variant < 0) return bfail(b, PARENT_E_UNSUPPORTED, "no tiled kernel fits these bins");
+    }
+    int tr, tc;
+    tile_shape(sb, used, variant, &tr, &tc);
+    if (b->queue_items != n || b->queue_method != used || b->queue_variant != variant) {
+        // Merged queue: items by decreasing frame count (cost per tile), each
+        // item's tiles in its band order, padded with empty tiles to a
+        // multiple of 8 and dealt so that block b of the launch runs entry b
+        // of this list whatever XCD it lands on (the kernels map block b to
+        // list slot (b % 8) * chunk + b / 8).
+        std::vector<int> order(n);
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return b->items[x]->traj.N > b->items[y]->traj.N; });
+        std::vector<int> lst;
+        for (int k : order) {
+            const Traj& t = b->items[k]->traj;
+            std::vector<int> tl = make_tiles(t.D, tr, tc);
+            for (size_t i = 0; i + 1 < tl.size(); i += 2) {
+                lst.push_back(tl[i]);
+                lst.push_back(tl[i + 1]);
+                lst.push_back(k);
+            }
+        }
+        while ((lst.size() / 3) % 8) {
+            const Traj& t0 = b->items[order[0]]->traj;
+            lst.push_back(t0.D);  // no valid pair: r0 = c0 = D
+            lst.push_back(t0.D);
+            lst.push_back(order[0]);
+        }
+        const int T = (int)(lst.size() / 3), chunk = T / 8;
+        if (T > (1 << 22)) return bfail(b, PARENT_E_UNSUPPORTED, "batch tile queue longer than one launch (4M tiles)");
+        std::vector<int> q(lst.size());
+        for (int blk = 0; blk < T; blk++) {
+            const int slot = (blk % 8) * chunk + blk / 8;
+            for (int c = 0; c < 3; c++) q[3 * slot + c] = lst[3 * blk + c];
+        }
+        std::vector<PairInputs> d(n);
+        for (int k = 0; k < n; k++) d[k] = with_nlogn(e, pair_inputs(b->items[k]->traj, b->items[k]->set));
+        HIPCHK(e, b->queue.ensure(q.size() * 4));
+        HIPCHK(e, b->descs.ensure(d.size() * sizeof(PairInputs)));
+        HIPCHK(e, hipMemcpyAsync(b->queue.p, q.data(), q.size() * 4, hipMemcpyHostToDevice, e->stream));
+        HIPCHK(e, hipMemcpyAsync(b->descs.p, d.data(), d.size() * sizeof(PairInputs), hipMemcpyHostToDevice, e->stream));
+        HIPCHK(e, hipStreamSynchronize(e->stream));
+        b->queue_items = n;
+        b->queue_method = used;
+        b->queue_variant = variant;
+        b->ntiles = T;
+    }
+    const PairInputs in0 = with_nlogn(e, pair_inputs(tb, sb));  // unused by the batch instantiations
+    const TileQueue q{b->queue.as<int>(), b->ntiles, 3, b->descs.as<PairInputs>()};
+    HIPCHK(e, hipEventRecord(e->ev[4], e->stream));
+    HIPCHK(e, launch_tiled(e, in0, bmax2d(sb), used, variant, q, nullptr, nullptr, nullptr));
+    HIPCHK(e, hipEventRecord(e->ev[5], e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    double upd = 0;
+    for (auto& it : b->items) {
+        it->set.s2d_valid = true;
+        it->set.method = used;
+        it->set.variant = variant;
+        upd += (double)n_pairs(it->traj.nD) * it->traj.N;
+    }
+    b->method = used;
+    b->variant = variant;
+    e->stats.ms_pairs = elapsed(e->ev[4], e->ev[5]);
+    e->stats.pair_updates = upd;
+    e->stats.method = used;
+    e->stats.kernel_variant = variant;
+    return PARENT_OK;
+}
+
+int parent_batch_read(parent_batch* b, int index, double* s1d, double* s2d) {
+    if (!b) return PARENT_E_INVALID;
+    parent_engine* e = b->eng;
+    if (index < 0 || index >= (int)b->items.size()) return fail(e, PARENT_E_INVALID, "batch index out of range");
+    HIPCHK(e, hipSetDevice(e->device));
+    BinSet& s = b->items[index]->set;
+    if (s1d)
+        if (int rc = read_s1d(e, s, s1d)) return rc;
+    if (s2d)
+        if (int rc = read_s2d(e, s, s2d)) return rc;
+    return PARENT_OK;
+}
+
+int parent_batch_mist(parent_batch* b, int index, int32_t* edges, double* mi, int32_t* left, double* totals) {
+    if (!b || !edges || !mi || !left || !totals) return PARENT_E_INVALID;
+    parent_engine* e = b->eng;
+    if (index < 0 || index >= (int)b->items.size()) return fail(e, PARENT_E_INVALID, "batch index out of range");
+    HIPCHK(e, hipSetDevice(e->device));
+    return run_mist(e, b->items[index]->set, edges, mi, left, totals);
+}
+
+int parent_batch_stats(parent_batch* b, parent_stats* out) {
+    if (!b || !out) return PARENT_E_INVALID;
+    *out = b->eng->stats;
     return PARENT_OK;
 }
 

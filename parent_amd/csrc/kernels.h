@@ -22,6 +22,8 @@ struct PrepareArgs {
     double* nlogn;                    // [N+1]
 };
 hipError_t launch_prepare(const PrepareArgs& a, hipStream_t s);
+// n ln n for n = 0..N (FP64 table of the fused epilogues)
+hipError_t launch_nlogn(int N, double* nlogn, hipStream_t s);
 
 // Common inputs of every pair kernel.
 struct PairInputs {
@@ -31,6 +33,17 @@ struct PairInputs {
     const double *vmin, *vmax, *jtab, *c2, *nlogn;
     double logN;
     double* s2d;          // [D(D-1)/2] .par order
+};
+
+// A work list of pair tiles.  One trajectory: tiles[t] = (r0, c0), stride 2,
+// descs == nullptr, the launch's PairInputs apply.  Batch of trajectories:
+// tiles[t] = (r0, c0, k), stride 3, descs[k] = trajectory k's inputs (device
+// memory, read once per workgroup).
+struct TileQueue {
+    const int* tiles;
+    int ntiles;
+    int stride;
+    const PairInputs* descs;
 };
 
 // Optional histogram dump of selected pairs (parity checks).
@@ -54,7 +67,7 @@ AtomicConfig atomic_config(int id);
 int atomic_kind(int id);  // 0 plain LDS histograms, 1 sliced (bank-conflict-free) histograms
 size_t atomic_lds_bytes(int id);
 bool atomic_fits(int id, int bmax, size_t lds_max);  // histograms of bmax x bmax fit its slots and the LDS
-hipError_t launch_pairs_atomic(const PairInputs& in, int config_id, const int* tiles, int ntiles,
+hipError_t launch_pairs_atomic(const PairInputs& in, int config_id, const TileQueue& q,
                                const int* dump_begin, const int* dump_entries, int* dump_out,
                                hipStream_t s);
 
@@ -62,7 +75,7 @@ hipError_t launch_pairs_atomic(const PairInputs& in, int config_id, const int* t
 // (0 when the 2D bin count is outside the kernel's range)
 int mfma_tile_dofs(int bmax);
 size_t mfma_lds_bytes();
-hipError_t launch_pairs_mfma(const PairInputs& in, int bmax, const int* tiles, int ntiles, const int* dump_begin,
+hipError_t launch_pairs_mfma(const PairInputs& in, int bmax, const TileQueue& q, const int* dump_begin,
                              const int* dump_entries, int* dump_out, hipStream_t s);
 
 // device Prim (MIST)

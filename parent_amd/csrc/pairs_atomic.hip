@@ -143,8 +143,8 @@ hipError_t launch_pairs_plain(const PairInputs& in, const CountsDump* dump, hipS
 // share one bin count (every tile off the diagonal band when the 2D bin counts
 // are equal) take a branch-free fill: per frame one 24-bit multiply per row
 // and one add + one LDS atomic per pair.
-template <int TR, int TC, int BLOCK, int SLOTW>
-__global__ __launch_bounds__(BLOCK) void k_pairs_atomic(PairInputs in, const int* __restrict__ tiles,
+template <int TR, int TC, int BLOCK, int SLOTW, bool BATCH>
+__global__ __launch_bounds__(BLOCK) void k_pairs_atomic(PairInputs in0, TileQueue q,
                                                         int t_begin, int t_end,
                                                         const int* __restrict__ dump_begin,
                                                         const int* __restrict__ dump_entries,
@@ -153,7 +153,9 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_atomic(PairInputs in, const int
     const int chunk = (t_end - t_begin + 7) / 8;
     const int t = t_begin + (int)(blockIdx.x % 8) * chunk + (int)(blockIdx.x / 8);
     if (t >= t_end || (int)(blockIdx.x / 8) >= chunk) return;
-    const int r0 = tiles[2 * t], c0 = tiles[2 * t + 1];
+    const int* tp = q.tiles + (size_t)(BATCH ? 3 : 2) * t;
+    const int r0 = tp[0], c0 = tp[1];
+    const PairInputs in = BATCH ? q.descs[tp[2]] : in0;
     const int D = in.D, N = in.N, nD = in.nD;
 
     int gr[TR], gc[TC], Bc[TC];
@@ -287,8 +289,8 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_atomic(PairInputs in, const int
 // serves in different cycles.  The count of a cell is the sum of its 16
 // sub-counters (epilogue).  64 bytes per cell: one 50 x 50 histogram fills
 // the 160 KB of LDS.
-template <int TR, int TC, int BLOCK, int SLOTC>
-__global__ __launch_bounds__(BLOCK, 1) void k_pairs_sliced(PairInputs in, const int* __restrict__ tiles,
+template <int TR, int TC, int BLOCK, int SLOTC, bool BATCH>
+__global__ __launch_bounds__(BLOCK, 1) void k_pairs_sliced(PairInputs in0, TileQueue q,
                                                         int t_begin, int t_end,
                                                         const int* __restrict__ dump_begin,
                                                         const int* __restrict__ dump_entries,
@@ -298,7 +300,9 @@ __global__ __launch_bounds__(BLOCK, 1) void k_pairs_sliced(PairInputs in, const 
     const int chunk = (t_end - t_begin + 7) / 8;
     const int t = t_begin + (int)(blockIdx.x % 8) * chunk + (int)(blockIdx.x / 8);
     if (t >= t_end || (int)(blockIdx.x / 8) >= chunk) return;
-    const int r0 = tiles[2 * t], c0 = tiles[2 * t + 1];
+    const int* tp = q.tiles + (size_t)(BATCH ? 3 : 2) * t;
+    const int r0 = tp[0], c0 = tp[1];
+    const PairInputs in = BATCH ? q.descs[tp[2]] : in0;
     const int D = in.D, N = in.N, nD = in.nD;
     const unsigned s16 = threadIdx.x & 15;
 
@@ -504,26 +508,33 @@ struct AtomicVariant {
     AtomicConfig cfg;
     int kind;
     int slot;  // words (kind 0) or cells (kind 1)
-    const void* fn;
 };
 
+// One launch of a configuration; batch queues (q.descs set) run the BATCH
+// instantiation, which reads each tile's inputs from q.descs.
 template <int KIND, int TR, int TC, int BLK, int SL>
-const void* atomic_fn() {
-    if constexpr (KIND == 0) return reinterpret_cast<const void*>(&k_pairs_atomic<TR, TC, BLK, SL>);
-    else return reinterpret_cast<const void*>(&k_pairs_sliced<TR, TC, BLK, SL>);
-}
-
-template <int KIND, int TR, int TC, int BLK, int SL>
-void launch_variant(dim3 grid, size_t lds, hipStream_t s, const PairInputs& in, const int* tiles, int t0,
-                    int t1, const int* db, const int* de, int* dout) {
+hipError_t launch_variant(dim3 grid, size_t lds, hipStream_t s, const PairInputs& in, const TileQueue& q, int t0,
+                          int t1, const int* db, const int* de, int* dout) {
+    const void* fn;
     if constexpr (KIND == 0)
-        k_pairs_atomic<TR, TC, BLK, SL><<<grid, BLK, lds, s>>>(in, tiles, t0, t1, db, de, dout);
+        fn = q.descs ? reinterpret_cast<const void*>(&k_pairs_atomic<TR, TC, BLK, SL, true>)
+                     : reinterpret_cast<const void*>(&k_pairs_atomic<TR, TC, BLK, SL, false>);
     else
-        k_pairs_sliced<TR, TC, BLK, SL><<<grid, BLK, lds, s>>>(in, tiles, t0, t1, db, de, dout);
+        fn = q.descs ? reinterpret_cast<const void*>(&k_pairs_sliced<TR, TC, BLK, SL, true>)
+                     : reinterpret_cast<const void*>(&k_pairs_sliced<TR, TC, BLK, SL, false>);
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    if constexpr (KIND == 0) {
+        if (q.descs) k_pairs_atomic<TR, TC, BLK, SL, true><<<grid, BLK, lds, s>>>(in, q, t0, t1, db, de, dout);
+        else k_pairs_atomic<TR, TC, BLK, SL, false><<<grid, BLK, lds, s>>>(in, q, t0, t1, db, de, dout);
+    } else {
+        if (q.descs) k_pairs_sliced<TR, TC, BLK, SL, true><<<grid, BLK, lds, s>>>(in, q, t0, t1, db, de, dout);
+        else k_pairs_sliced<TR, TC, BLK, SL, false><<<grid, BLK, lds, s>>>(in, q, t0, t1, db, de, dout);
+    }
+    return hipGetLastError();
 }
 
-#define PARENT_ATOMIC_ENTRY(ID, K, TR, TC, BLK, SL) \
-    AtomicVariant{AtomicConfig{TR, TC, BLK}, K, SL, atomic_fn<K, TR, TC, BLK, SL>()},
+#define PARENT_ATOMIC_ENTRY(ID, K, TR, TC, BLK, SL) AtomicVariant{AtomicConfig{TR, TC, BLK}, K, SL},
 const AtomicVariant kAtomicVariants[] = {PARENT_ATOMIC_VARIANTS(PARENT_ATOMIC_ENTRY)};
 #undef PARENT_ATOMIC_ENTRY
 
@@ -540,30 +551,30 @@ bool atomic_fits(int id, int bmax, size_t lds_max) {
     return (long long)bmax * bmax <= kAtomicVariants[id].slot && atomic_lds_bytes(id) <= lds_max;
 }
 
-hipError_t launch_pairs_atomic(const PairInputs& in, int id, const int* tiles, int ntiles,
+hipError_t launch_pairs_atomic(const PairInputs& in, int id, const TileQueue& q,
                                const int* dump_begin, const int* dump_entries, int* dump_out,
                                hipStream_t s) {
     if (id < 0 || id >= atomic_config_count()) return hipErrorInvalidValue;
     const size_t lds = atomic_lds_bytes(id);
-    hipError_t e = hipFuncSetAttribute(kAtomicVariants[id].fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return e;
+    const int ntiles = q.ntiles;
     const int per_launch = 1 << 22;
     for (int t0 = 0; t0 < ntiles; t0 += per_launch) {
         const int t1 = t0 + per_launch < ntiles ? t0 + per_launch : ntiles;
         const int chunk = (t1 - t0 + 7) / 8;
         dim3 grid((unsigned)chunk * 8);
+        hipError_t e = hipErrorInvalidValue;
         switch (id) {
-#define PARENT_ATOMIC_CASE(ID, K, TR, TC, BLK, SL)                                              \
-    case ID:                                                                                     \
-        launch_variant<K, TR, TC, BLK, SL>(grid, lds, s, in, tiles, t0, t1, dump_begin,         \
-                                           dump_entries, dump_out);                              \
+#define PARENT_ATOMIC_CASE(ID, K, TR, TC, BLK, SL)                                                           \
+    case ID:                                                                                                  \
+        e = launch_variant<K, TR, TC, BLK, SL>(grid, lds, s, in, q, t0, t1, dump_begin, dump_entries, dump_out); \
         break;
             PARENT_ATOMIC_VARIANTS(PARENT_ATOMIC_CASE)
 #undef PARENT_ATOMIC_CASE
-            default: return hipErrorInvalidValue;
+            default: break;
         }
+        if (e != hipSuccess) return e;
     }
-    return hipGetLastError();
+    return hipSuccess;
 }
 
 }  // namespace parent

@@ -57,7 +57,8 @@ __device__ __forceinline__ int wave_sum_i(int v) {
 
 }  // namespace
 
-__global__ __launch_bounds__(256, 2) void k_pairs_mfma(PairInputs in, int B, int TD, const int* __restrict__ tiles,
+template <bool BATCH>
+__global__ __launch_bounds__(256, 2) void k_pairs_mfma(PairInputs in0, int B, int TD, TileQueue q,
                                                        int t_begin, int t_end, const int* __restrict__ dump_begin,
                                                        const int* __restrict__ dump_entries, int* __restrict__ dump_out) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
@@ -66,7 +67,9 @@ __global__ __launch_bounds__(256, 2) void k_pairs_mfma(PairInputs in, int B, int
     const int chunk_t = (t_end - t_begin + 7) / 8;
     const int t = t_begin + (int)(blockIdx.x % 8) * chunk_t + (int)(blockIdx.x / 8);
     if (t >= t_end || (int)(blockIdx.x / 8) >= chunk_t) return;
-    const int r0 = tiles[2 * t], c0 = tiles[2 * t + 1];
+    const int* tp = q.tiles + (size_t)(BATCH ? 3 : 2) * t;
+    const int r0 = tp[0], c0 = tp[1];
+    const PairInputs in = BATCH ? q.descs[tp[2]] : in0;
     const int D = in.D, N = in.N, nD = in.nD;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int wr = wid >> 1, wc = wid & 1;
@@ -224,18 +227,23 @@ int mfma_tile_dofs(int bins) {
 
 size_t mfma_lds_bytes() { return kLds; }
 
-hipError_t launch_pairs_mfma(const PairInputs& in, int bins, const int* tiles, int ntiles, const int* dump_begin,
+hipError_t launch_pairs_mfma(const PairInputs& in, int bins, const TileQueue& q, const int* dump_begin,
                              const int* dump_entries, int* dump_out, hipStream_t s) {
+    const int ntiles = q.ntiles;
     const int TD = mfma_tile_dofs(bins);
     if (TD < 1) return hipErrorInvalidValue;
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_pairs_mfma),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, kLds);
+    const void* fn = q.descs ? reinterpret_cast<const void*>(&k_pairs_mfma<true>)
+                             : reinterpret_cast<const void*>(&k_pairs_mfma<false>);
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, kLds);
     if (e != hipSuccess) return e;
     const int per_launch = 1 << 22;
     for (int t0 = 0; t0 < ntiles; t0 += per_launch) {
         const int t1 = t0 + per_launch < ntiles ? t0 + per_launch : ntiles;
         const int chunk = (t1 - t0 + 7) / 8;
-        k_pairs_mfma<<<chunk * 8, 256, kLds, s>>>(in, bins, TD, tiles, t0, t1, dump_begin, dump_entries, dump_out);
+        if (q.descs)
+            k_pairs_mfma<true><<<chunk * 8, 256, kLds, s>>>(in, bins, TD, q, t0, t1, dump_begin, dump_entries, dump_out);
+        else
+            k_pairs_mfma<false><<<chunk * 8, 256, kLds, s>>>(in, bins, TD, q, t0, t1, dump_begin, dump_entries, dump_out);
     }
     return hipGetLastError();
 }

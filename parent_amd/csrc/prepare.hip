@@ -445,4 +445,9 @@ hipError_t launch_prepare(const PrepareArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
+hipError_t launch_nlogn(int N, double* nlogn, hipStream_t s) {
+    k_nlogn<<<(N + 256) / 256, 256, 0, s>>>(N, nlogn);
+    return hipGetLastError();
+}
+
 }  // namespace parent

@@ -1,0 +1,146 @@
+"""Batched paths on the GPU: the bin-count sweep (one captured hipGraph per set
+of bin counts) and the batch of uneven trajectories (one launch over a merged
+tile queue).  Bars as in test_gpu_parity.py: every replay of the graph equals
+an independent eager engine bit for bit; batch results match the oracle
+(|dS2D| <= 1e-10, MI within 1e-6 relative with a 1e-11 floor) and the
+single-trajectory engine's S1D bit for bit.
+"""
+import numpy as np
+import pytest
+
+import oracle_api as O
+
+pytestmark = pytest.mark.gpu
+
+MI_REL, MI_ABS, S_ABS = 1e-6, 1e-11, 1e-10
+
+SWEEP = [5, 10, 20, 30, 40, 50, 75, 100, [7, 13, 29, 5, 40, 17]]
+
+
+def _pa():
+    import parent_amd
+    return parent_amd
+
+
+def _rows(bin_sets):
+    return [[b] * 6 if np.ndim(b) == 0 else list(b) for b in bin_sets]
+
+
+def _mi(nD, s1, s2):
+    D = 3 * nD + 3
+    gi, gj = np.triu_indices(D, 1)
+    pos = np.array([O.lib().oracle_par_pos(nD, int(a), int(b)) for a, b in zip(gi, gj)])
+    return s1[gi] + s1[gj] - s2[pos]
+
+
+def _check_vs_oracle(nD, vals, bins, s1, s2):
+    s1o, s2o, *_ = O.compute(nD, vals.shape[0], vals.T.astype(np.float64), bins)
+    assert np.max(np.abs(s1 - s1o)) <= 1e-12
+    assert np.max(np.abs(s2 - s2o)) <= S_ABS
+    mi, mio = _mi(nD, s1, s2), _mi(nD, s1o, s2o)
+    err = np.abs(mi - mio)
+    assert np.all(err <= MI_REL * np.abs(mio) + MI_ABS), f"max MI err {err.max():.3e}"
+
+
+def test_sweep_replays_match_eager_runs():
+    """The sweep's graph, launched three times with eager kernels (MIST,
+    pair_counts) and host syncs between the launches: every launch equals an
+    independent engine that prepares and pairs each bin set eagerly.  The MFMA
+    request makes the kernel choice deterministic (MFMA up to 64 bins, the
+    default atomic configuration above), so both engines run the same kernels."""
+    pa = _pa()
+    vals = pa.synth_values(22, 4100, 31)
+    nD = 22 - 3
+    rows = _rows(SWEEP)
+    print(f"sweep: {len(rows)} bin sets, eager references", flush=True)
+    ref = []
+    with pa.Engine(0) as eager:
+        eager.set_trajectory(vals)
+        for b in rows:
+            eager.prepare(b)
+            eager.pairs(pa.METHOD_MFMA)
+            ref.append((eager.s1d(), eager.s2d(), eager.stats()["method"]))
+            edges, mi, left, tot = eager.mist()
+            ref[-1] = ref[-1] + (tot,)
+    with pa.Engine(0) as eng:
+        eng.set_trajectory(vals)
+        for launch in range(3):
+            print(f"sweep launch {launch}", flush=True)
+            eng.sweep(rows, pa.METHOD_MFMA)
+            st = eng.stats()
+            assert st["sweep_sets"] == len(rows) and st["ms_sweep"] > 0
+            for k, (s1r, s2r, _, totr) in enumerate(ref):
+                eng.select(k)
+                s1, s2 = eng.s1d(), eng.s2d()
+                assert np.array_equal(s1.view(np.uint64), s1r.view(np.uint64)), f"set {k} S1D, launch {launch}"
+                assert np.array_equal(s2.view(np.uint64), s2r.view(np.uint64)), f"set {k} S2D, launch {launch}"
+            # eager work between the launches: MIST on two sets, counts on one
+            for k in (0, len(rows) - 1):
+                eng.select(k)
+                _, _, _, tot = eng.mist()
+                assert np.array_equal(tot.view(np.uint64), ref[k][3].view(np.uint64))
+            eng.select(1)
+            h = eng.pair_counts([(0, 1), (2, 3 * nD + 2)], pa.METHOD_PLAIN, rows[1][3:])
+            assert all(x.sum() == vals.shape[0] for x in h)
+    methods = [m for _, _, m, _ in ref]
+    assert methods[0] == pa.METHOD_MFMA and methods[-2] == pa.METHOD_ATOMIC  # 5 bins / 100 bins
+
+
+def test_sweep_against_oracle():
+    pa = _pa()
+    vals = pa.synth_values(14, 3000, 5)
+    nD = 11
+    rows = _rows([5, 50, [10, 10, 10, 20, 30, 40]])
+    with pa.Engine(0) as eng:
+        eng.set_trajectory(vals)
+        eng.sweep(rows, pa.METHOD_AUTO)
+        for k, b in enumerate(rows):
+            print(f"sweep vs oracle: bins {b}", flush=True)
+            eng.select(k)
+            _check_vs_oracle(nD, vals, b, eng.s1d(), eng.s2d())
+
+
+@pytest.mark.parametrize("bins,method", [([30] * 6, 3), ([10] * 6, 2), ([7, 13, 29, 5, 40, 17], 1),
+                                         ([50] * 6, 0)])
+def test_batch_of_uneven_trajectories(bins, method):
+    """Trajectories of different dof and frame counts in one launch: each
+    trajectory's S1D equals the single engine's bit for bit, S2D and MI
+    match the oracle; MIST totals equal the single engine's."""
+    pa = _pa()
+    shapes = [(10, 1500, 1), (23, 4097, 2), (17, 800, 3), (40, 3000, 4), (6, 5000, 5)]
+    trajs = [pa.synth_values(a, n, s) for a, n, s in shapes]
+    print(f"batch bins={bins} method={method}: {len(trajs)} trajectories", flush=True)
+    with pa.Batch(bins) as b, pa.Engine(0) as single:
+        for v in trajs:
+            b.add(v)
+        b.run(method)
+        st = b.stats()
+        assert st["pair_updates"] == sum((3 * a - 6) * (3 * a - 7) // 2 * n for a, n, _ in shapes)
+        for k, v in enumerate(trajs):
+            nD = v.shape[1] // 3 - 1
+            s1, s2 = b.results(k)
+            single.set_trajectory(v)
+            single.prepare(bins)
+            assert np.array_equal(s1.view(np.uint64), single.s1d().view(np.uint64))
+            _check_vs_oracle(nD, v, bins, s1, s2)
+            single.pairs(pa.METHOD_PLAIN)
+            s2p = single.s2d()
+            assert np.max(np.abs(s2 - s2p)) <= S_ABS
+            _, _, _, tot = b.mist(k)
+            _, _, _, tot1 = single.mist()
+            assert np.max(np.abs(tot - tot1)) <= 1e-8
+
+
+def test_batch_rejects_bad_data():
+    pa = _pa()
+    v = pa.synth_values(10, 500, 2)
+    bad = v.copy()
+    bad[7, 3] = -0.5
+    with pa.Batch([20] * 6) as b:
+        b.add(v)
+        with pytest.raises(pa.ParentError) as ei:
+            b.add(bad)
+        assert ei.value.code == 4
+        b.run(pa.METHOD_AUTO)  # the good trajectory still runs
+        s1, s2 = b.results(0)
+        assert np.all(np.isfinite(s2))
