@@ -309,6 +309,12 @@ def run_batch(args, pa, nthreads):
         del v
     log(f"{len(shapes)} trajectories synthesised, uploaded and pre-binned in {time.time() - t0:.1f} s "
         f"(device pre-binning {prep_ms:.1f} ms); {updates:.4e} pair-frame updates")
+    tune = None
+    if args.tune_report:
+        tune = b.autotune()
+        for t in tune:
+            log(f"batch tune: method {t['method']} variant {t['variant']} tile {t['tile_rows']}x{t['tile_cols']} "
+                f"block {t['block']}: predicted {t['ms']:.1f} ms")
     for _ in range(args.warmup):
         b.run(method)
     run_ms = []
@@ -333,7 +339,7 @@ def run_batch(args, pa, nthreads):
           "atoms_frames": shapes, "method": st["method"], "variant": st["kernel_variant"]},
          {"phases": {"prepare_ms_total": prep_ms, "pairs_ms": pair_avg,
                      "pairs_kernel_updates_per_s": updates / (pair_avg / 1e3)},
-          "mist_totals": mist, "roofline": roofline, "cpu_baseline": cpu})
+          "mist_totals": mist, "roofline": roofline, "cpu_baseline": cpu, "tuning": tune})
 
 
 def main():

@@ -255,6 +255,13 @@ int parent_batch_add(parent_batch* b, int n_dihedrals, int n_frames, int double_
  * no tile queue).  stats.ms_pairs is that launch. */
 int parent_batch_run(parent_batch* b, int method);
 
+/* The batch autotuner (what PARENT_METHOD_AUTO runs on first use): every
+ * configuration that fits is timed on the tuning rows of the longest and the
+ * shortest trajectory, the two timings fit ms = a * tiles + b * updates, and
+ * the configuration with the lowest predicted time for the whole merged
+ * queue is kept.  Returns the table (entries[k].ms = predicted ms). */
+int parent_batch_autotune(parent_batch* b, parent_tune_entry* entries, int max_entries, int* n_entries);
+
 /* S1D / S2D of trajectory `index` (either pointer may be NULL). */
 int parent_batch_read(parent_batch* b, int index, double* s1d, double* s2d);
 

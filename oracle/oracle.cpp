@@ -393,6 +393,33 @@ int oracle_compute(int nD, int N, double* values, const int bins[6], double* s1d
     return 0;
 }
 
+int oracle_compute_subset(int nD, int N, int n_sel, const int32_t* dofs, double* cols, const int bins[6],
+                          int n_pairs, const int32_t* pairs, double* s1d_sel, double* s2d_pairs, int nthreads) {
+    const double pi = acos(-1);  // PARENT.cpp:112
+    std::vector<double> vmin(n_sel), vmax(n_sel), mf(n_sel);
+    int err = 0;
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1) reduction(max : err)
+    for (int k = 0; k < n_sel; k++) {
+        int r = oracle_dof_range(type_of(nD, dofs[k]), N, cols + (size_t)k * N, pi, &vmin[k], &vmax[k], &mf[k]);
+        if (r > err) err = r;
+    }
+    if (err) return err;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int k = 0; k < n_sel; k++) {
+        const int t = type_of(nD, dofs[k]);
+        s1d_sel[k] = oracle_entropy1d(t, N, cols + (size_t)k * N, vmin[k], vmax[k], bins[t]);
+    }
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int p = 0; p < n_pairs; p++) {
+        const int a = pairs[2 * p], b = pairs[2 * p + 1];
+        const int t1 = type_of(nD, dofs[a]), t2 = type_of(nD, dofs[b]);
+        s2d_pairs[p] = oracle_entropy2d(t1, cols + (size_t)a * N, vmin[a], vmax[a], bins[3 + t1], t2,
+                                        cols + (size_t)b * N, vmin[b], vmax[b], bins[3 + t2], N, nullptr);
+    }
+    return 0;
+}
+
 double oracle_mutual(int nD, const double* s1d, const double* s2d, int gi, int gj) {
     int lo = gi < gj ? gi : gj, hi = gi < gj ? gj : gi;
     return s1d[lo] + s1d[hi] - s2d[oracle_par_pos(nD, lo, hi)];

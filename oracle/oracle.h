@@ -77,6 +77,17 @@ int oracle_compute(int n_dihedrals, int n_frames, double* values, const int bins
                    double* s1d, double* s2d, double* vmin, double* vmax,
                    double* modfit, int nthreads);
 
+/* The same computation restricted to a selection of dofs and a list of
+ * pairs among them, for problems too large to run every pair on the host.
+ * cols: dof-major [n_sel][N] FP64 values of the selected global dofs
+ * dofs[0..n_sel) (shifted in place for dihedrals); pairs: [n_pairs][2]
+ * indices into the selection, dofs[a] < dofs[b].  s1d_sel: [n_sel];
+ * s2d_pairs: [n_pairs].  Returns 0 or the first nonzero oracle_dof_range
+ * code. */
+int oracle_compute_subset(int n_dihedrals, int n_frames, int n_sel, const int32_t* dofs, double* cols,
+                          const int bins[6], int n_pairs, const int32_t* pairs, double* s1d_sel,
+                          double* s2d_pairs, int nthreads);
+
 /* MI of a pair from .par arrays (util.cpp:47-81): (S1D[lo] + S1D[hi]) - S2D. */
 double oracle_mutual(int n_dihedrals, const double* s1d, const double* s2d, int gi, int gj);
 

@@ -74,6 +74,7 @@ def lib() -> C.CDLL:
         "parent_batch_read": (i32, [p, i32, dp, dp]),
         "parent_batch_mist": (i32, [p, i32, p, dp, p, dp]),
         "parent_batch_stats": (i32, [p, p]),
+        "parent_batch_autotune": (i32, [p, p, i32, C.POINTER(i32)]),
         "parent_synth_values": (i32, [i32, i32, u64, p, i32]),
         "parent_synth_write_bat": (i32, [C.c_char_p, i32, i32, u64, i32]),
     }
@@ -394,6 +395,13 @@ class Batch:
 
     def run(self, method: int = METHOD_AUTO) -> None:
         self._check(lib().parent_batch_run(self._h, method))
+
+    def autotune(self) -> list:
+        """The batch autotuner's table (ms = predicted time of the whole queue)."""
+        arr = (_TuneEntry * 64)()
+        n = C.c_int()
+        self._check(lib().parent_batch_autotune(self._h, C.cast(arr, C.c_void_p), 64, C.byref(n)))
+        return [{k: getattr(arr[i], k) for k, _ in _TuneEntry._fields_} for i in range(min(n.value, 64))]
 
     def results(self, k: int):
         """(s1d, s2d) of trajectory k."""

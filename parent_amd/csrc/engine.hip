@@ -16,6 +16,7 @@
 #include "kernels.h"
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -145,6 +146,7 @@ struct parent_batch {
     DevBuf queue, descs;
     int queue_items = -1, queue_method = -1, queue_variant = -1, ntiles = 0;
     int method = PARENT_METHOD_PLAIN, variant = 0;
+    int tuned_items = -1, tuned_method = -1, tuned_variant = -1;  // batch autotuner's choice
 };
 
 namespace {
@@ -302,6 +304,23 @@ std::vector<int> make_tiles(int D, int tr, int tc) {
     const int band = std::max(tr, 16 / tr * tr);
     for (int b0 = 0; b0 < D - 1; b0 += band) push_band(t, D, tr, tc, b0, std::min(b0 + band, D - 1));
     return t;
+}
+
+// Number of tiles make_tiles(D, tr, tc) lists, without building the list.
+long long count_tiles(int D, int tr, int tc) {
+    long long n = 0;
+    const int band = std::max(tr, 16 / tr * tr);
+    for (int b0 = 0; b0 < D - 1; b0 += band) {
+        const int b1 = std::min(b0 + band, D - 1);
+        const int cfirst = ((b0 + 1) / tc) * tc;
+        for (int r0 = b0; r0 < b1; r0 += tr) {
+            // columns c0 = cfirst + k tc < D with c0 + tc - 1 > r0
+            const int lo = r0 - tc + 2;
+            const int c_start = cfirst >= lo ? cfirst : cfirst + (lo - cfirst + tc - 1) / tc * tc;
+            if (c_start < D) n += (D - c_start + tc - 1) / tc;
+        }
+    }
+    return n;
 }
 
 // Tile shape (row dofs x column dofs) of a tiled method's variant.
@@ -614,6 +633,83 @@ int create_stream_and_events(parent_engine* e, int device) {
 }
 
 int bfail(parent_batch* b, int code, const std::string& msg) { return fail(b->eng, code, msg); }
+
+// Kernel choice for a whole batch.  One configuration runs every tile of the
+// merged queue, and the right one depends on the mix: per pair, a tile costs
+// a fixed part (clearing its LDS histograms, the epilogue) and a part per
+// frame.  Each candidate is timed on the tuning rows of the batch's longest
+// and shortest trajectories; the two timings give the model
+// ms = alpha * tiles + beta * updates, which predicts the full queue.
+int batch_autotune(parent_engine* e, parent_batch* b, int* method, int* variant) {
+    const int n = (int)b->items.size();
+    int kl = 0, ks = 0;
+    for (int k = 1; k < n; k++) {
+        const Traj& t = b->items[k]->traj;
+        if (t.N > b->items[kl]->traj.N || (t.N == b->items[kl]->traj.N && t.D > b->items[kl]->traj.D)) kl = k;
+        if (t.N < b->items[ks]->traj.N || (t.N == b->items[ks]->traj.N && t.D > b->items[ks]->traj.D)) ks = k;
+    }
+    BinSet& s0 = b->items[kl]->set;
+    std::vector<std::pair<int, int>> cand;
+    for (int id = 0; id < atomic_config_count(); id++)
+        if (variant_fits(e, s0, id)) cand.push_back({PARENT_METHOD_ATOMIC, id});
+    if (mfma_fits(e, s0)) cand.push_back({PARENT_METHOD_MFMA, 0});
+    if (cand.empty()) return bfail(b, PARENT_E_UNSUPPORTED, "no tiled kernel fits these bins");
+    double total_upd = 0;
+    for (auto& it : b->items) total_upd += (double)n_pairs(it->traj.nD) * it->traj.N;
+    e->last_tune.clear();
+    DevBuf dt;
+    double best = -1;
+    for (auto& mv : cand) {
+        const int m = mv.first, id = mv.second;
+        int tr, tc;
+        tile_shape(s0, m, id, &tr, &tc);
+        double ms[2] = {0, 0}, tiles[2] = {0, 0}, upd[2] = {0, 0};
+        const int probes[2] = {kl, ks};
+        const int np = kl == ks ? 1 : 2;
+        for (int p = 0; p < np; p++) {
+            const Traj& t = b->items[probes[p]]->traj;
+            BinSet& s = b->items[probes[p]]->set;
+            std::vector<int> w = tune_work(t.D, tr, tc, tune_rows(t.D));
+            if (w.empty()) continue;
+            tiles[p] = (double)(w.size() / 2);
+            upd[p] = (double)tile_pairs(t.D, w, tr, tc) * t.N;
+            HIPCHK(e, dt.ensure(w.size() * sizeof(int)));
+            HIPCHK(e, hipMemcpyAsync(dt.p, w.data(), w.size() * sizeof(int), hipMemcpyHostToDevice, e->stream));
+            const PairInputs in = with_nlogn(e, pair_inputs(t, s));
+            const TileQueue q{dt.as<int>(), (int)(w.size() / 2), 2, nullptr};
+            if (int rc = time_launch(e, [&] { return launch_tiled(e, in, bmax2d(s), m, id, q, nullptr, nullptr, nullptr); }, &ms[p]))
+                return rc;
+        }
+        double alpha = 0, beta = 0;
+        const double det = tiles[0] * upd[1] - tiles[1] * upd[0];
+        const bool fit = np == 2 && std::fabs(det) > 1e-9 * std::fabs(tiles[0] * upd[1]);
+        if (fit) {
+            alpha = (ms[0] * upd[1] - ms[1] * upd[0]) / det;
+            beta = (tiles[0] * ms[1] - tiles[1] * ms[0]) / det;
+        }
+        if (!fit || alpha < 0 || beta < 0) {
+            // no separable fit: charge both timings to one term
+            if (fit && beta < 0 && alpha >= 0) {
+                beta = 0;
+                alpha = (ms[0] + ms[1]) / (tiles[0] + tiles[1]);
+            } else {
+                alpha = 0;
+                beta = (ms[0] + ms[1]) / (upd[0] + upd[1]);
+            }
+        }
+        double total_tiles = 0;
+        for (auto& it : b->items) total_tiles += (double)count_tiles(it->traj.D, tr, tc);
+        const double pred = alpha * total_tiles + beta * total_upd;
+        const int block = m == PARENT_METHOD_MFMA ? 256 : atomic_config(id).block;
+        e->last_tune.push_back(parent_tune_entry{m, id, tr, tc, block, pred, total_upd, total_upd / (pred * 1e-3)});
+        if (best < 0 || pred < best) {
+            best = pred;
+            *method = m;
+            *variant = id;
+        }
+    }
+    return PARENT_OK;
+}
 
 }  // namespace
 
@@ -1065,7 +1161,16 @@ int parent_batch_run(parent_batch* b, int method) {
     BinSet& sb = b->items[big]->set;
     const Traj& tb = b->items[big]->traj;
     int used = method, variant = 0;
-    if (int rc = resolve_method(e, tb, sb, method, &used, &variant)) return rc;
+    if (method == PARENT_METHOD_AUTO) {
+        if (b->tuned_items != n) {
+            if (int rc = batch_autotune(e, b, &b->tuned_method, &b->tuned_variant)) return rc;
+            b->tuned_items = n;
+        }
+        used = b->tuned_method;
+        variant = b->tuned_variant;
+    } else if (int rc = resolve_method(e, tb, sb, method, &used, &variant)) {
+        return rc;
+    }
     if (used == PARENT_METHOD_PLAIN) {
         // the plain kernel has no tile queue: the tiled atomic kernel runs instead
         used = PARENT_METHOD_ATOMIC;
@@ -1099,12 +1204,18 @@ int parent_batch_run(parent_batch* b, int method) {
             lst.push_back(t0.D);
             lst.push_back(order[0]);
         }
-        const int T = (int)(lst.size() / 3), chunk = T / 8;
-        if (T > (1 << 22)) return bfail(b, PARENT_E_UNSUPPORTED, "batch tile queue longer than one launch (4M tiles)");
+        if (lst.size() / 3 > (size_t)INT32_MAX) return bfail(b, PARENT_E_UNSUPPORTED, "batch tile queue too long");
+        const int T = (int)(lst.size() / 3);
+        // the launchers issue the queue in launches of `seg` tiles (a multiple
+        // of 8), each with its own block-to-slot map: deal each segment alone
+        const int seg = used == PARENT_METHOD_MFMA ? mfma_tiles_per_launch() : atomic_tiles_per_launch(variant);
         std::vector<int> q(lst.size());
-        for (int blk = 0; blk < T; blk++) {
-            const int slot = (blk % 8) * chunk + blk / 8;
-            for (int c = 0; c < 3; c++) q[3 * slot + c] = lst[3 * blk + c];
+        for (int s0 = 0; s0 < T; s0 += seg) {
+            const int n_seg = std::min(seg, T - s0), chunk = n_seg / 8;
+            for (int blk = 0; blk < n_seg; blk++) {
+                const int slot = s0 + (blk % 8) * chunk + blk / 8;
+                for (int c = 0; c < 3; c++) q[3 * (size_t)slot + c] = lst[3 * ((size_t)s0 + blk) + c];
+            }
         }
         std::vector<PairInputs> d(n);
         for (int k = 0; k < n; k++) d[k] = with_nlogn(e, pair_inputs(b->items[k]->traj, b->items[k]->set));
@@ -1137,6 +1248,23 @@ int parent_batch_run(parent_batch* b, int method) {
     e->stats.pair_updates = upd;
     e->stats.method = used;
     e->stats.kernel_variant = variant;
+    return PARENT_OK;
+}
+
+int parent_batch_autotune(parent_batch* b, parent_tune_entry* entries, int max_entries, int* n_entries) {
+    if (!b) return PARENT_E_INVALID;
+    parent_engine* e = b->eng;
+    if (b->items.empty()) return bfail(b, PARENT_E_INVALID, "empty batch");
+    HIPCHK(e, hipSetDevice(e->device));
+    int nmax = 0;
+    for (auto& it : b->items) nmax = std::max(nmax, it->traj.N);
+    HIPCHK(e, e->scratch.nlogn.ensure(((size_t)nmax + 1) * 8));
+    HIPCHK(e, launch_nlogn(nmax, e->scratch.nlogn.as<double>(), e->stream));
+    if (int rc = batch_autotune(e, b, &b->tuned_method, &b->tuned_variant)) return rc;
+    b->tuned_items = (int)b->items.size();
+    const int n = (int)e->last_tune.size();
+    if (n_entries) *n_entries = n;
+    for (int k = 0; k < n && k < max_entries && entries; k++) entries[k] = e->last_tune[k];
     return PARENT_OK;
 }
 

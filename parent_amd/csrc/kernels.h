@@ -71,6 +71,17 @@ hipError_t launch_pairs_atomic(const PairInputs& in, int config_id, const TileQu
                                const int* dump_begin, const int* dump_entries, int* dump_out,
                                hipStream_t s);
 
+// Tiles per launch of a tiled kernel with `block` threads: a multiple of 8
+// (the XCD block-to-slot map) with grid x block < 2^32 (the dispatch packet
+// counts work-items in 32 bits), at most 2^22.  Longer queues are issued as
+// several launches of this many tiles.
+inline int tiles_per_launch(int block) {
+    const unsigned cap = (0xFFFFFFFFu / (unsigned)block) & ~7u;
+    return cap < (1u << 22) ? (int)cap : (1 << 22);
+}
+int atomic_tiles_per_launch(int config_id);
+int mfma_tiles_per_launch();
+
 // one-hot int8 MFMA kernel: square tiles of mfma_tile_dofs(bmax) dofs a side
 // (0 when the 2D bin count is outside the kernel's range)
 int mfma_tile_dofs(int bmax);

@@ -551,13 +551,15 @@ bool atomic_fits(int id, int bmax, size_t lds_max) {
     return (long long)bmax * bmax <= kAtomicVariants[id].slot && atomic_lds_bytes(id) <= lds_max;
 }
 
+int atomic_tiles_per_launch(int id) { return tiles_per_launch(kAtomicVariants[id].cfg.block); }
+
 hipError_t launch_pairs_atomic(const PairInputs& in, int id, const TileQueue& q,
                                const int* dump_begin, const int* dump_entries, int* dump_out,
                                hipStream_t s) {
     if (id < 0 || id >= atomic_config_count()) return hipErrorInvalidValue;
     const size_t lds = atomic_lds_bytes(id);
     const int ntiles = q.ntiles;
-    const int per_launch = 1 << 22;
+    const int per_launch = atomic_tiles_per_launch(id);
     for (int t0 = 0; t0 < ntiles; t0 += per_launch) {
         const int t1 = t0 + per_launch < ntiles ? t0 + per_launch : ntiles;
         const int chunk = (t1 - t0 + 7) / 8;

@@ -227,6 +227,8 @@ int mfma_tile_dofs(int bins) {
 
 size_t mfma_lds_bytes() { return kLds; }
 
+int mfma_tiles_per_launch() { return tiles_per_launch(256); }
+
 hipError_t launch_pairs_mfma(const PairInputs& in, int bins, const TileQueue& q, const int* dump_begin,
                              const int* dump_entries, int* dump_out, hipStream_t s) {
     const int ntiles = q.ntiles;
@@ -236,7 +238,7 @@ hipError_t launch_pairs_mfma(const PairInputs& in, int bins, const TileQueue& q,
                              : reinterpret_cast<const void*>(&k_pairs_mfma<false>);
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, kLds);
     if (e != hipSuccess) return e;
-    const int per_launch = 1 << 22;
+    const int per_launch = mfma_tiles_per_launch();
     for (int t0 = 0; t0 < ntiles; t0 += per_launch) {
         const int t1 = t0 + per_launch < ntiles ? t0 + per_launch : ntiles;
         const int chunk = (t1 - t0 + 7) / 8;

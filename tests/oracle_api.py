@@ -31,6 +31,7 @@ def lib():
             "oracle_entropy2d": (dp, [i32, p, dp, dp, i32, i32, p, dp, dp, i32, i32, p]),
             "oracle_par_pos": (C.c_int64, [i32, i32, i32]),
             "oracle_compute": (i32, [i32, i32, p, p, p, p, p, p, p, i32]),
+            "oracle_compute_subset": (i32, [i32, i32, i32, p, p, p, i32, p, p, p, i32]),
             "oracle_mutual": (dp, [i32, p, p, i32, i32]),
             "oracle_mist": (i32, [i32, p, p, p, p, p, p]),
         }
@@ -106,6 +107,22 @@ def compute(nD, N, values_dof_major, bins, nthreads=0):
     if rc:
         raise RuntimeError(f"oracle_compute failed: {rc}")
     return s1, s2, mn, mx, mf, v
+
+
+def compute_subset(nD, N, dofs, cols_dof_major, bins, pairs, nthreads=0):
+    """S1D of the selected dofs and S2D of pairs among them -> (s1d_sel, s2d_pairs).
+    cols_dof_major: [len(dofs)][N] values; pairs: [n][2] indices into dofs."""
+    d = np.ascontiguousarray(dofs, dtype=np.int32)
+    v = np.array(cols_dof_major, dtype=np.float64, order="C", copy=True)
+    pr = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1, 2)
+    b = np.ascontiguousarray(bins, dtype=np.int32)
+    s1 = np.empty(d.size)
+    s2 = np.empty(pr.shape[0])
+    rc = lib().oracle_compute_subset(nD, N, d.size, _p(d), _p(v), _p(b), pr.shape[0], _p(pr), _p(s1), _p(s2),
+                                     nthreads)
+    if rc:
+        raise RuntimeError(f"oracle_compute_subset failed: {rc}")
+    return s1, s2
 
 
 def mist(nD, s1d, s2d):

@@ -252,3 +252,22 @@ def test_oracle_reproduces_ubq_sample_totals(tmp_path):
     edges, mi, left, tot = O.mist(nD, s1, s2)
     mist_tail = open(os.path.join(GOLD, "ubq_sample_MIST_tail.txt")).read().splitlines()
     assert mist_tail[-1] == f"TOTAL CONFIGURATIONAL ENTROPY = {_fmt(tot[9])}"
+
+
+def test_subset_computation_equals_full():
+    """oracle_compute_subset on a dof selection gives the bits of oracle_compute."""
+    rng = np.random.default_rng(5)
+    nD, N = 7, 1500
+    D = 3 * nD + 3
+    v = np.empty((D, N))
+    v[: nD + 2] = rng.normal(0.13, 0.003, (nD + 2, N))
+    v[nD + 2: 2 * nD + 3] = rng.uniform(1.8, 2.1, (nD + 1, N))
+    v[2 * nD + 3:] = rng.uniform(0.0, 6.28, (nD, N))
+    bins = [20, 30, 40, 25, 35, 45]
+    s1, s2, *_ = O.compute(nD, N, v, bins)
+    dofs = np.array([0, 3, nD + 2, nD + 5, 2 * nD + 3, D - 1], dtype=np.int32)
+    pairs = [(a, b) for a in range(dofs.size) for b in range(a + 1, dofs.size)]
+    s1s, s2s = O.compute_subset(nD, N, dofs, v[dofs], bins, pairs)
+    assert np.array_equal(s1s, s1[dofs])
+    pos = [O.lib().oracle_par_pos(nD, int(dofs[a]), int(dofs[b])) for a, b in pairs]
+    assert np.array_equal(s2s, s2[pos])

@@ -144,3 +144,22 @@ def test_batch_rejects_bad_data():
         b.run(pa.METHOD_AUTO)  # the good trajectory still runs
         s1, s2 = b.results(0)
         assert np.all(np.isfinite(s2))
+
+
+def test_queue_longer_than_one_launch():
+    """4.48M one-pair tiles on a 1,024-thread configuration: the queue is
+    issued as two launches (grid x block must stay below 2^32 work-items);
+    every pair matches the plain kernel."""
+    pa = _pa()
+    vals = pa.synth_values(1000, 600, 7)  # 2,994 dofs, 4,480,521 pairs
+    with pa.Engine(0) as eng:
+        eng.set_trajectory(vals)
+        eng.prepare([20] * 6)
+        eng.pairs(pa.METHOD_PLAIN)
+        s2p = eng.s2d()
+        eng.set_variant(pa.METHOD_ATOMIC, 19)  # sliced 1 x 1 tiles, 1,024 threads
+        print("long queue: variant 19", flush=True)
+        eng.pairs(pa.METHOD_ATOMIC)
+        assert eng.stats()["method"] == pa.METHOD_ATOMIC and eng.stats()["kernel_variant"] == 19
+        s2 = eng.s2d()
+        assert np.max(np.abs(s2 - s2p)) <= S_ABS
