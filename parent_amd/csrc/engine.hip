@@ -350,8 +350,8 @@ int ensure_tiles(parent_engine* e, const Traj& t, BinSet& s, int method, int var
     return PARENT_OK;
 }
 
-bool variant_fits(const parent_engine* e, const BinSet& s, int id) {
-    return atomic_fits(id, bmax2d(s), (size_t)e->lds_max);
+bool variant_fits(const parent_engine* e, const BinSet& s, int id, int n_frames) {
+    return atomic_fits(id, bmax2d(s), (size_t)e->lds_max, n_frames);
 }
 
 bool mfma_fits(const parent_engine* e, const BinSet& s) {
@@ -381,13 +381,13 @@ long long tile_pairs(int D, const std::vector<int>& t, int tr, int tc) {
     return n;
 }
 
-int choose_atomic_config(const parent_engine* e, const BinSet& s) {
-    if (e->forced_atomic >= 0) return variant_fits(e, s, e->forced_atomic) ? e->forced_atomic : -1;
+int choose_atomic_config(const parent_engine* e, const BinSet& s, int n_frames) {
+    if (e->forced_atomic >= 0) return variant_fits(e, s, e->forced_atomic, n_frames) ? e->forced_atomic : -1;
     // fallback order when not autotuning: the best of the autotuner's tables
     // at 50 bins first, then configurations with larger histogram slots
     static const int order[] = {2, 3, 1, 9, 0, 11, 13, 10, 15, 16, 14, 18};
     for (int id : order)
-        if (variant_fits(e, s, id)) return id;
+        if (variant_fits(e, s, id, n_frames)) return id;
     return -1;
 }
 
@@ -434,7 +434,7 @@ int run_autotune(parent_engine* e, const Traj& tj, BinSet& s) {
     // candidates: every fitting atomic configuration, then the MFMA kernel
     std::vector<std::pair<int, int>> cand;
     for (int id = 0; id < atomic_config_count(); id++)
-        if (variant_fits(e, s, id)) cand.push_back({PARENT_METHOD_ATOMIC, id});
+        if (variant_fits(e, s, id, tj.N)) cand.push_back({PARENT_METHOD_ATOMIC, id});
     if (mfma_fits(e, s)) cand.push_back({PARENT_METHOD_MFMA, 0});
     for (auto& mv : cand) {
         const int m = mv.first, id = mv.second;
@@ -489,7 +489,7 @@ int resolve_method(parent_engine* e, const Traj& tj, BinSet& s, int method, int*
         *used = PARENT_METHOD_ATOMIC;
     }
     if (*used == PARENT_METHOD_ATOMIC) {
-        *variant = choose_atomic_config(e, s);
+        *variant = choose_atomic_config(e, s, tj.N);
         if (*variant < 0) {
             *used = PARENT_METHOD_PLAIN;
             *variant = 0;
@@ -651,7 +651,7 @@ int batch_autotune(parent_engine* e, parent_batch* b, int* method, int* variant)
     BinSet& s0 = b->items[kl]->set;
     std::vector<std::pair<int, int>> cand;
     for (int id = 0; id < atomic_config_count(); id++)
-        if (variant_fits(e, s0, id)) cand.push_back({PARENT_METHOD_ATOMIC, id});
+        if (variant_fits(e, s0, id, b->items[kl]->traj.N)) cand.push_back({PARENT_METHOD_ATOMIC, id});
     if (mfma_fits(e, s0)) cand.push_back({PARENT_METHOD_MFMA, 0});
     if (cand.empty()) return bfail(b, PARENT_E_UNSUPPORTED, "no tiled kernel fits these bins");
     double total_upd = 0;
@@ -1161,6 +1161,7 @@ int parent_batch_run(parent_batch* b, int method) {
     BinSet& sb = b->items[big]->set;
     const Traj& tb = b->items[big]->traj;
     int used = method, variant = 0;
+    if (method < PARENT_METHOD_PLAIN || method > PARENT_METHOD_AUTO) return bfail(b, PARENT_E_INVALID, "unknown method");
     if (method == PARENT_METHOD_AUTO) {
         if (b->tuned_items != n) {
             if (int rc = batch_autotune(e, b, &b->tuned_method, &b->tuned_variant)) return rc;
@@ -1168,13 +1169,15 @@ int parent_batch_run(parent_batch* b, int method) {
         }
         used = b->tuned_method;
         variant = b->tuned_variant;
-    } else if (int rc = resolve_method(e, tb, sb, method, &used, &variant)) {
-        return rc;
-    }
-    if (used == PARENT_METHOD_PLAIN) {
-        // the plain kernel has no tile queue: the tiled atomic kernel runs instead
+    } else if (method == PARENT_METHOD_MFMA && mfma_fits(e, sb)) {
+        used = PARENT_METHOD_MFMA;
+        variant = 0;
+    } else {
+        // ATOMIC, MFMA outside its bin range, and PLAIN (which has no tile
+        // queue): the tiled atomic kernel, in a configuration whose counters
+        // hold the longest trajectory
         used = PARENT_METHOD_ATOMIC;
-        variant = choose_atomic_config(e, sb);
+        variant = choose_atomic_config(e, sb, nmax);
         if (variant < 0) return bfail(b, PARENT_E_UNSUPPORTED, "no tiled kernel fits these bins");
     }
     int tr, tc;

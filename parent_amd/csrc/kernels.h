@@ -64,9 +64,13 @@ struct AtomicConfig {
 };
 int atomic_config_count();
 AtomicConfig atomic_config(int id);
-int atomic_kind(int id);  // 0 plain LDS histograms, 1 sliced (bank-conflict-free) histograms
+int atomic_kind(int id);  // 0 plain LDS histograms, 1 sliced (bank-conflict-free), 2 packed sliced
 size_t atomic_lds_bytes(int id);
-bool atomic_fits(int id, int bmax, size_t lds_max);  // histograms of bmax x bmax fit its slots and the LDS
+// Packed sliced histograms keep 16-bit counters of the frames of one lane
+// position (N / 16 + 16 of them at most).
+constexpr int kPackedMaxFrames = (65535 - 16) * 16;
+// histograms of bmax x bmax fit its slots and the LDS, and its counters hold n_frames
+bool atomic_fits(int id, int bmax, size_t lds_max, int n_frames);
 hipError_t launch_pairs_atomic(const PairInputs& in, int config_id, const TileQueue& q,
                                const int* dump_begin, const int* dump_entries, int* dump_out,
                                hipStream_t s);

@@ -289,14 +289,26 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_atomic(PairInputs in0, TileQueu
 // serves in different cycles.  The count of a cell is the sum of its 16
 // sub-counters (epilogue).  64 bytes per cell: one 50 x 50 histogram fills
 // the 160 KB of LDS.
-template <int TR, int TC, int BLOCK, int SLOTC, bool BATCH>
+//
+// PACK = 2 halves that: the cells (2k, b) and (2k+1, b) share their 16
+// words as 16-bit halves (word = ((a/2) * Bj + b) * 16 + s; a lane adds 1
+// or 65536 by the row's parity, so the increment is shared by every column
+// of a tile row).  The bank pattern is unchanged, so the fill stays
+// conflict-free, and a 50 x 50 histogram takes 80 KB, two workgroups per CU.
+// A half counts at most the frames of one lane position, N/16 + 16, so
+// PACK = 2 needs N <= kPackedMaxFrames.
+//
+// The fill computes byte offsets straight from the index bytes (a 24-bit
+// multiply with a byte operand for the row, a shift for the column, one
+// three-way add), four frames per index word.
+template <int TR, int TC, int BLOCK, int SLOTC, int PACK, bool BATCH>
 __global__ __launch_bounds__(BLOCK, 1) void k_pairs_sliced(PairInputs in0, TileQueue q,
                                                         int t_begin, int t_end,
                                                         const int* __restrict__ dump_begin,
                                                         const int* __restrict__ dump_entries,
                                                         int* __restrict__ dump_out) {
     extern __shared__ unsigned hist[];
-    constexpr int kSlotW = SLOTC * 16;
+    constexpr int kSlotW = SLOTC / PACK * 16;
     const int chunk = (t_end - t_begin + 7) / 8;
     const int t = t_begin + (int)(blockIdx.x % 8) * chunk + (int)(blockIdx.x / 8);
     if (t >= t_end || (int)(blockIdx.x / 8) >= chunk) return;
@@ -341,43 +353,78 @@ __global__ __launch_bounds__(BLOCK, 1) void k_pairs_sliced(PairInputs in0, TileQ
 #pragma unroll
     for (int c = 0; c < TC; c++) cols[c] = in.bins + (size_t)(gc[c] < D ? gc[c] : 0) * in.Np;
 
+    // word of cell (a, b) in a slot with Bj columns, and the lane's increment
+    auto cell_word = [&](unsigned a, unsigned b, unsigned Bj) {
+        if constexpr (PACK == 1) return ((__umul24(a, Bj) + b) << 4) + s16;
+        else return ((__umul24(a >> 1, Bj) + b) << 4) + s16;
+    };
+    auto cell_inc = [&](unsigned a) {
+        if constexpr (PACK == 1) return 1u;
+        else return 1u << ((a & 1u) << 4);
+    };
+    unsigned char* hb = reinterpret_cast<unsigned char*>(hist);
+    constexpr int kSlotB = kSlotW * 4;
+    auto comp = [](const uint4& v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : (k == 2 ? v.z : v.w)); };
     int it0 = 0;
     if (fast && nfull > 0) {
-        const unsigned B16 = (unsigned)Bc[0] * 16u;
-        // software pipeline: the next iteration's index words are loaded
-        // before this iteration's atomics are issued
-        uint4 Rn[TR], Cn[TC];
+        // byte offset of row byte a: (a or a & ~1) * rowmul; of column byte b: b * 64
+        const unsigned rowmul = (unsigned)Bc[0] * (PACK == 1 ? 64u : 32u);
+        const unsigned s4 = s16 * 4u;
+        auto process = [&](const uint4* R, const uint4* C) {
 #pragma unroll
-        for (int r = 0; r < TR; r++) Rn[r] = *reinterpret_cast<const uint4*>(rows[r] + threadIdx.x * 16);
+            for (int k = 0; k < 4; k++) {
+                unsigned rw[TR], rsh[TR], cw[TC];
 #pragma unroll
-        for (int c = 0; c < TC; c++) Cn[c] = *reinterpret_cast<const uint4*>(cols[c] + threadIdx.x * 16);
-        for (int it = 0; it < nfull; it++) {
-            uint4 R[TR], C[TC];
+                for (int r = 0; r < TR; r++) {
+                    const unsigned w = comp(R[r], k);
+                    rw[r] = PACK == 1 ? w : (w & 0xFEFEFEFEu);
+                    rsh[r] = (w & 0x01010101u) << 4;
+                }
 #pragma unroll
-            for (int r = 0; r < TR; r++) R[r] = Rn[r];
+                for (int c = 0; c < TC; c++) cw[c] = comp(C[c], k);
 #pragma unroll
-            for (int c = 0; c < TC; c++) C[c] = Cn[c];
-            const int fn = (it + 1) * per_iter + threadIdx.x * 16;
-            if (fn < in.Np) {
+                for (int e = 0; e < 4; e++) {
+                    unsigned ro[TR], inc[TR], co[TC];
 #pragma unroll
-                for (int r = 0; r < TR; r++) Rn[r] = *reinterpret_cast<const uint4*>(rows[r] + fn);
+                    for (int r = 0; r < TR; r++) {
+                        ro[r] = __umul24((rw[r] >> (8 * e)) & 0xFFu, rowmul) + s4;
+                        inc[r] = PACK == 1 ? 1u : 1u << ((rsh[r] >> (8 * e)) & 0xFFu);
+                    }
 #pragma unroll
-                for (int c = 0; c < TC; c++) Cn[c] = *reinterpret_cast<const uint4*>(cols[c] + fn);
+                    for (int c = 0; c < TC; c++) co[c] = ((cw[c] >> (8 * e)) & 0xFFu) << 6;
+#pragma unroll
+                    for (int r = 0; r < TR; r++)
+#pragma unroll
+                        for (int c = 0; c < TC; c++)
+                            atomicAdd(reinterpret_cast<unsigned*>(hb + (r * TC + c) * kSlotB + ro[r] + co[c]), inc[r]);
+                }
             }
+        };
+        // two register sets: the next iteration's index words load while
+        // this iteration's atomics issue, without copies between the sets
+        auto load = [&](uint4* R, uint4* C, int it) {
+            const int f = it * per_iter + threadIdx.x * 16;
+            if (f < in.Np) {
 #pragma unroll
-            for (int e = 0; e < 16; e++) {
-                unsigned ro[TR], co[TC];
+                for (int r = 0; r < TR; r++) R[r] = *reinterpret_cast<const uint4*>(rows[r] + f);
 #pragma unroll
-                for (int r = 0; r < TR; r++) ro[r] = __umul24(byte_of(R[r], e), B16) + s16;
-#pragma unroll
-                for (int c = 0; c < TC; c++) co[c] = byte_of(C[c], e) << 4;
-#pragma unroll
-                for (int r = 0; r < TR; r++)
-#pragma unroll
-                    for (int c = 0; c < TC; c++) atomicAdd(&hist[(r * TC + c) * kSlotW + ro[r] + co[c]], 1u);
+                for (int c = 0; c < TC; c++) C[c] = *reinterpret_cast<const uint4*>(cols[c] + f);
             }
+        };
+        uint4 R0[TR], C0[TC], R1[TR], C1[TC];
+        load(R0, C0, 0);
+        int it = 0;
+        for (; it + 2 <= nfull; it += 2) {
+            load(R1, C1, it + 1);
+            process(R0, C0);
+            load(R0, C0, it + 2);
+            process(R1, C1);
         }
-        it0 = nfull;
+        if (it < nfull) {
+            process(R0, C0);
+            it++;
+        }
+        it0 = it;
     }
     for (int it = it0; it < niter; it++) {
         const int f = it * per_iter + threadIdx.x * 16;
@@ -395,22 +442,30 @@ __global__ __launch_bounds__(BLOCK, 1) void k_pairs_sliced(PairInputs in0, TileQ
                 const unsigned a = byte_of(R[r], e);
 #pragma unroll
                 for (int c = 0; c < TC; c++) {
-                    if (valid & (1u << (r * TC + c)))
-                        atomicAdd(&hist[(r * TC + c) * kSlotW + ((__umul24(a, (unsigned)Bc[c]) + byte_of(C[c], e)) << 4) + s16], 1u);
+                    if (valid & (1u << (r * TC + c))) {
+                        const unsigned b = byte_of(C[c], e);
+                        atomicAdd(&hist[(r * TC + c) * kSlotW + cell_word(a, b, (unsigned)Bc[c])], cell_inc(a));
+                    }
                 }
             }
         }
     }
     __syncthreads();
 
-    // count of cell i of slot sl: sum of its 16 sub-counters
-    auto cell_count = [&](int sl, int i) {
-        const uint4* q = reinterpret_cast<const uint4*>(hist + sl * kSlotW + i * 16);
+    // count of cell i (row-major over Bj columns) of slot sl: sum of its 16 sub-counters
+    auto cell_count = [&](int sl, int i, int Bj) {
+        const unsigned a = (unsigned)(i / Bj), b = (unsigned)(i - (int)a * Bj);
+        const uint4* q = reinterpret_cast<const uint4*>(hist + sl * kSlotW + (cell_word(a, b, (unsigned)Bj) - s16));
         unsigned n = 0;
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             const uint4 v = q[k];
-            n += v.x + v.y + v.z + v.w;
+            if constexpr (PACK == 1) {
+                n += v.x + v.y + v.z + v.w;
+            } else {
+                const unsigned sh = (a & 1u) * 16u;
+                n += ((v.x >> sh) & 0xFFFFu) + ((v.y >> sh) & 0xFFFFu) + ((v.z >> sh) & 0xFFFFu) + ((v.w >> sh) & 0xFFFFu);
+            }
         }
         return n;
     };
@@ -421,7 +476,7 @@ __global__ __launch_bounds__(BLOCK, 1) void k_pairs_sliced(PairInputs in0, TileQ
             const long long off = ((long long)dump_entries[3 * e + 1] << 31) | (long long)dump_entries[3 * e + 2];
             const int r = sl / TC, c = sl % TC;
             const int cells = in.B.of(dof_type(nD, gr[r])) * Bc[c];
-            for (int i = threadIdx.x; i < cells; i += BLOCK) dump_out[off + i] = (int)cell_count(sl, i);
+            for (int i = threadIdx.x; i < cells; i += BLOCK) dump_out[off + i] = (int)cell_count(sl, i, Bc[c]);
         }
         return;
     }
@@ -431,16 +486,37 @@ __global__ __launch_bounds__(BLOCK, 1) void k_pairs_sliced(PairInputs in0, TileQ
         if (!(valid & (1u << sl))) continue;
         const int r = sl / TC, c = sl % TC;
         const int gi = gr[r], gj = gc[c];
-        const int cells = in.B.of(dof_type(nD, gi)) * Bc[c];
+        const int Bi = in.B.of(dof_type(nD, gi)), Bj = Bc[c];
         // all waves share a slot: per-wave partial sums, then wave 0 combines
         __shared__ double sT[kWaves];
         __shared__ int sOcc[kWaves];
         double T = 0.0;
         int occ = 0;
-        for (int i = threadIdx.x; i < cells; i += BLOCK) {
-            const unsigned n = cell_count(sl, i);
-            if (n) {
-                T += in.nlogn[n];
+        // Word groups (the 16 sub-counter words of a cell, or of a cell pair
+        // when packed), one per thread: thread t reads word (s + t) % 16 of
+        // its group at step s, so the 16 lanes served together hit 16
+        // distinct banks.
+        const int ngroups = (PACK == 1 ? Bi : (Bi + 1) / 2) * Bj;
+        const unsigned* hs = hist + sl * kSlotW;
+        for (int g = threadIdx.x; g < ngroups; g += BLOCK) {
+            unsigned lo = 0, hi = 0;
+#pragma unroll
+            for (int k = 0; k < 16; k++) {
+                const unsigned w = hs[g * 16 + ((k + threadIdx.x) & 15)];
+                if constexpr (PACK == 1) {
+                    lo += w;
+                } else {
+                    lo += w & 0xFFFFu;
+                    hi += w >> 16;
+                }
+            }
+            if (lo) {
+                T += in.nlogn[lo];
+                occ++;
+            }
+            // the odd row of a packed pair exists unless Bi is odd and this is the last pair row
+            if (PACK == 2 && hi && 2 * (g / Bj) + 1 < Bi) {
+                T += in.nlogn[hi];
                 occ++;
             }
         }
@@ -470,7 +546,9 @@ namespace {
 // histograms, slot size in words (2560 -> B <= 50, 5632 -> B <= 75,
 // 10240 -> B <= 101, 40960 -> B <= 202); kind 1: sliced histograms, slot size
 // in cells (64 bytes each; 2500 -> B <= 50, 1225 -> B <= 35, 900 -> B <= 30,
-// 400 -> B <= 20, 100 -> B <= 10).
+// 400 -> B <= 20, 100 -> B <= 10); kind 2: packed sliced histograms, slot
+// size in cells of B x (B rounded up to even) (32 bytes each; 2500 -> B <= 50,
+// 3600 -> B <= 60, 1296 -> B <= 36, 900 -> B <= 30).
 #define PARENT_ATOMIC_VARIANTS(X)     \
     X(0, 0, 1, 1, 256, 2560)          \
     X(1, 0, 2, 2, 256, 2560)          \
@@ -502,7 +580,16 @@ namespace {
     X(27, 1, 1, 4, 1024, 400)         \
     X(28, 1, 1, 1, 256, 400)          \
     X(29, 1, 2, 2, 512, 100)          \
-    X(30, 1, 4, 4, 1024, 100)
+    X(30, 1, 4, 4, 1024, 100)         \
+    X(31, 2, 1, 1, 512, 2500)         \
+    X(32, 2, 1, 1, 1024, 2500)        \
+    X(33, 2, 1, 1, 256, 2500)         \
+    X(34, 2, 1, 2, 1024, 2500)        \
+    X(35, 2, 1, 1, 512, 3600)         \
+    X(36, 2, 1, 1, 512, 1296)         \
+    X(37, 2, 1, 2, 512, 1296)         \
+    X(38, 2, 2, 2, 1024, 900)         \
+    X(39, 2, 1, 1, 256, 900)
 
 struct AtomicVariant {
     AtomicConfig cfg;
@@ -515,21 +602,22 @@ struct AtomicVariant {
 template <int KIND, int TR, int TC, int BLK, int SL>
 hipError_t launch_variant(dim3 grid, size_t lds, hipStream_t s, const PairInputs& in, const TileQueue& q, int t0,
                           int t1, const int* db, const int* de, int* dout) {
+    constexpr int PACK = KIND == 2 ? 2 : 1;
     const void* fn;
     if constexpr (KIND == 0)
         fn = q.descs ? reinterpret_cast<const void*>(&k_pairs_atomic<TR, TC, BLK, SL, true>)
                      : reinterpret_cast<const void*>(&k_pairs_atomic<TR, TC, BLK, SL, false>);
     else
-        fn = q.descs ? reinterpret_cast<const void*>(&k_pairs_sliced<TR, TC, BLK, SL, true>)
-                     : reinterpret_cast<const void*>(&k_pairs_sliced<TR, TC, BLK, SL, false>);
+        fn = q.descs ? reinterpret_cast<const void*>(&k_pairs_sliced<TR, TC, BLK, SL, PACK, true>)
+                     : reinterpret_cast<const void*>(&k_pairs_sliced<TR, TC, BLK, SL, PACK, false>);
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     if constexpr (KIND == 0) {
         if (q.descs) k_pairs_atomic<TR, TC, BLK, SL, true><<<grid, BLK, lds, s>>>(in, q, t0, t1, db, de, dout);
         else k_pairs_atomic<TR, TC, BLK, SL, false><<<grid, BLK, lds, s>>>(in, q, t0, t1, db, de, dout);
     } else {
-        if (q.descs) k_pairs_sliced<TR, TC, BLK, SL, true><<<grid, BLK, lds, s>>>(in, q, t0, t1, db, de, dout);
-        else k_pairs_sliced<TR, TC, BLK, SL, false><<<grid, BLK, lds, s>>>(in, q, t0, t1, db, de, dout);
+        if (q.descs) k_pairs_sliced<TR, TC, BLK, SL, PACK, true><<<grid, BLK, lds, s>>>(in, q, t0, t1, db, de, dout);
+        else k_pairs_sliced<TR, TC, BLK, SL, PACK, false><<<grid, BLK, lds, s>>>(in, q, t0, t1, db, de, dout);
     }
     return hipGetLastError();
 }
@@ -545,10 +633,14 @@ AtomicConfig atomic_config(int id) { return kAtomicVariants[id].cfg; }
 int atomic_kind(int id) { return kAtomicVariants[id].kind; }
 size_t atomic_lds_bytes(int id) {
     const AtomicVariant& v = kAtomicVariants[id];
-    return (size_t)v.cfg.tr * v.cfg.tc * v.slot * (v.kind == 1 ? 64 : 4);
+    const size_t per_slot = v.kind == 0 ? (size_t)v.slot * 4 : (v.kind == 1 ? (size_t)v.slot * 64 : (size_t)v.slot / 2 * 64);
+    return (size_t)v.cfg.tr * v.cfg.tc * per_slot;
 }
-bool atomic_fits(int id, int bmax, size_t lds_max) {
-    return (long long)bmax * bmax <= kAtomicVariants[id].slot && atomic_lds_bytes(id) <= lds_max;
+bool atomic_fits(int id, int bmax, size_t lds_max, int n_frames) {
+    const AtomicVariant& v = kAtomicVariants[id];
+    const long long cells = v.kind == 2 ? (long long)bmax * ((bmax + 1) & ~1) : (long long)bmax * bmax;
+    if (v.kind == 2 && n_frames > kPackedMaxFrames) return false;
+    return cells <= v.slot && atomic_lds_bytes(id) <= lds_max;
 }
 
 int atomic_tiles_per_launch(int id) { return tiles_per_launch(kAtomicVariants[id].cfg.block); }
