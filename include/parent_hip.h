@@ -97,8 +97,21 @@ int parent_par_write(const char* path, const parent_bat* header, const int bins[
 int parent_par_read(const char* path, int* n_dihedrals, int* n_frames, int bins[6],
                     double* s1d, double* s2d);
 
-/* Message of the last failed parent_bat_* / parent_par_* / parent_synth_*
- * call on this thread. */
+/* Writes the MIE listing of a .par body, byte-identical to the stdout of
+ * the reference's get_PAR_MIE.x (get_PAR_MIE.cpp:54-199): the 1D entropies,
+ * one "<t1>-<t2> i  j   S2D   MI" line per pair, then the 16 totals, all
+ * at precision 12.  Replaces that program's listing and summing loops;
+ * MI follows get_mutual (util.cpp:47-81).
+ *   path      output file, "-" for stdout, or NULL to compute totals only
+ *   s1d, s2d  the .par body (parent_par_read layout)
+ *   nthreads  formatting threads (<= 0: OpenMP default)
+ *   totals    NULL or 16 doubles, in the order of the printed TOTAL lines
+ *             (each a sequential FP64 sum in listing order) */
+int parent_mie_write(const char* path, int n_dihedrals, const double* s1d, const double* s2d,
+                     int nthreads, double* totals);
+
+/* Message of the last failed parent_bat_* / parent_par_* / parent_synth_* /
+ * parent_mie_* call on this thread. */
 const char* parent_last_error(void);
 
 /* ------------------------------------------------------------------ */

@@ -10,6 +10,7 @@ functions here follow them:
 
 * :func:`run_parent` -- ``PARENT.x in.bat out.par b1 a1 d1 b2 a2 d2``
 * :func:`run_mist`   -- ``get_PAR_MIST.x in.par out.txt``
+* :func:`mie_write`  -- ``get_PAR_MIE.x in.par > out_MIE.txt``
 * :func:`get_mutual` -- ``get_mutual`` (util.cpp:47-81)
 """
 from .api import (  # noqa: F401
@@ -25,6 +26,7 @@ from .api import (  # noqa: F401
     get_mutual,
     lib,
     library_path,
+    mie_write,
     par_pos,
     read_par,
     run_mist,

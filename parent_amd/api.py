@@ -77,6 +77,7 @@ def lib() -> C.CDLL:
         "parent_batch_autotune": (i32, [p, p, i32, C.POINTER(i32)]),
         "parent_synth_values": (i32, [i32, i32, u64, p, i32]),
         "parent_synth_write_bat": (i32, [C.c_char_p, i32, i32, u64, i32]),
+        "parent_mie_write": (i32, [C.c_char_p, i32, dp, dp, i32, dp]),
     }
     del i64
     for name, (res, args) in sig.items():
@@ -199,6 +200,21 @@ def read_par(path: str):
     s2 = np.empty(D * (D - 1) // 2, dtype=np.float64)
     _check_host(L.parent_par_read(os.fsencode(path), C.byref(nD), C.byref(N), _ptr(bins), _ptr(s1), _ptr(s2)))
     return nD.value, N.value, bins, s1, s2
+
+
+def mie_write(path, n_dihedrals: int, s1d: np.ndarray, s2d: np.ndarray, nthreads: int = 0) -> np.ndarray:
+    """Writes the get_PAR_MIE.x listing of a .par body to `path` ("-" for
+    stdout, None for totals only); returns the 16 totals in the order of the
+    listing's TOTAL lines (get_PAR_MIE.cpp:109-199)."""
+    s1 = np.ascontiguousarray(s1d, dtype=np.float64)
+    s2 = np.ascontiguousarray(s2d, dtype=np.float64)
+    _, _, _, D = dof_counts(n_dihedrals)
+    if s1.size != D or s2.size != D * (D - 1) // 2:
+        raise ValueError("s1d / s2d sizes do not match n_dihedrals")
+    tot = np.empty(16, dtype=np.float64)
+    _check_host(lib().parent_mie_write(os.fsencode(path) if path is not None else None, n_dihedrals,
+                                       _ptr(s1), _ptr(s2), nthreads, _ptr(tot)))
+    return tot
 
 
 def synth_values(atoms: int, n_frames: int, seed: int, nthreads: int = 0) -> np.ndarray:

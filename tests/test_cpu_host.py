@@ -182,3 +182,92 @@ def test_get_mutual_and_par_pos_match_oracle():
             seen.add(p)
             assert pa.get_mutual(nD, s1, s2, i, j) == O.lib().oracle_mutual(nD, O._p(s1), O._p(s2), i, j)
     assert seen == set(range(D * (D - 1) // 2))
+
+
+MIE_EXE = os.path.join(ROOT, "parent_amd", "bin", "get_PAR_MIE_hip.x")
+
+
+def test_mie_listing_matches_reference_golden(tmp_path):
+    """get_PAR_MIE_hip.x on the golden .par prints the bytes the reference's
+    get_PAR_MIE.x printed for it (chain12_b50_MIE.txt)."""
+    r = subprocess.run([MIE_EXE, os.path.join(GOLD, "chain12_b50.par")], capture_output=True)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout == open(os.path.join(GOLD, "chain12_b50_MIE.txt"), "rb").read()
+
+
+def _planted_par(path, atoms, seed, specials):
+    """A .par of an `atoms` chain whose S1D/S2D are planted values spanning
+    magnitudes, signs, 12-digit rounding ties and (optionally) inf/nan."""
+    bat = str(path) + ".bat"
+    pa.synth_write_bat(bat, atoms, 1, 1, 1)
+    nD = atoms - 3
+    _, _, _, D = pa.dof_counts(nD)
+    rng = np.random.default_rng(seed)
+    P = D * (D - 1) // 2
+    def draw(n):
+        v = rng.standard_normal(n) * 10.0 ** rng.integers(-12, 12, n)
+        k = rng.integers(0, 8, n)
+        v[k == 0] = np.round(v[k == 0], 3)            # short decimals
+        v[k == 1] = rng.integers(-10**6, 10**6, (k == 1).sum()) * 1.0
+        v[k == 2] = 0.5 + rng.integers(0, 10**11, (k == 2).sum()) * 1e-12 + 5e-13  # 12th-digit ties
+        return v
+    s1, s2 = draw(D), draw(P)
+    if specials:
+        s2[:6] = [np.inf, -np.inf, np.nan, -0.0, 5e-324, 1.7976931348623157e308]
+        s1[0] = -np.nan
+    with pa.Bat(bat) as b:
+        b.write_par(str(path), [50] * 6, s1, s2)
+    return nD, s1, s2
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(O.REF_DIR, "get_PAR_MIE.x")),
+                    reason="reference get_PAR_MIE.x not built (needs /root/reference)")
+@pytest.mark.parametrize("atoms,specials", [(40, True), (100, False)])
+def test_mie_listing_matches_reference_binary(tmp_path, atoms, specials):
+    """Byte comparison with the reference get_PAR_MIE.x (built from its
+    sources) on planted values, including inf, nan, -0 and denormals."""
+    par = tmp_path / "x.par"
+    _planted_par(par, atoms, atoms, specials)
+    ref = subprocess.run([os.path.join(O.REF_DIR, "get_PAR_MIE.x"), str(par)], capture_output=True)
+    ours = subprocess.run([MIE_EXE, str(par)], capture_output=True, env={**os.environ, "OMP_NUM_THREADS": "3"})
+    assert ref.returncode == 0 and ours.returncode == 0, (ref.stderr, ours.stderr)
+    assert ours.stdout == ref.stdout
+
+
+def test_mie_totals_are_sequential_listing_sums(tmp_path):
+    """The 16 totals: each TOTAL line's value, and the category sums equal
+    sequential FP64 sums over the listed values (order fixed by the listing,
+    which the byte comparisons pin)."""
+    par = tmp_path / "t.par"
+    nD, s1, s2 = _planted_par(par, 30, 3, False)
+    out = tmp_path / "t.txt"
+    tot = pa.mie_write(str(out), nD, s1, s2, 2)
+    lines = open(out).read().splitlines()
+    totals = [float(x.split(" = ")[1]) for x in lines[-16:]]
+    assert np.allclose(totals, tot, rtol=1e-11, atol=0)
+    nB, nA, nd, D = pa.dof_counts(nD)
+    assert len(lines) == D + D * (D - 1) // 2 + 16
+    acc = 0.0
+    for v in s1[:nB]:
+        acc += v
+    assert acc == tot[0]
+    # bond-bond block: entry k of the listing is bb[P-1-k]
+    bb = s2[:nB * (nB - 1) // 2][::-1]
+    acc_e = acc_m = 0.0
+    k = 0
+    for i in range(nB - 1):
+        for j in range(i + 1, nB):
+            acc_e += bb[k]
+            acc_m += (s1[i] + s1[j]) - bb[k]
+            k += 1
+    assert (acc_e, acc_m) == (tot[3], tot[4])
+    assert tot[15] == (tot[0] + tot[1] + tot[2] - tot[4] - tot[6] - tot[8] - tot[10] - tot[12] - tot[14])
+
+
+def test_mie_cli_errors_exit_1(tmp_path):
+    r = subprocess.run([MIE_EXE, str(tmp_path / "missing.par")], capture_output=True, text=True)
+    assert r.returncode == 1 and "COULD NOT OPEN FILE" in r.stderr
+    bad = tmp_path / "bad.par"
+    bad.write_bytes(struct.pack("<10i", 4, 2, 3, 10, 50, 50, 50, 50, 50, 50))
+    r = subprocess.run([MIE_EXE, str(bad)], capture_output=True, text=True)
+    assert r.returncode == 1 and "ERROR READING HEADER OF FILE" in r.stderr
