@@ -17,6 +17,10 @@ Workloads (--workload):
   batch64_b30             64 chains, atoms log-uniform in [50, 2000], frames log-uniform in
                           [5k, 200k] (seed 4), 30 bins: one step = the pre-binning of every
                           trajectory (timed on the device) + one pair launch over the merged queue
+  readin_1500x100k        read-in of the default trajectory as a 1.8 GB FP32 .bat file (page-cached):
+                          one step = parent_engine_load_bat (file -> pinned buffers -> HBM ->
+                          frame-major values); reported in bytes/s beside the host path
+                          (parent_bat_read_frames + parent_engine_set_trajectory)
 
 Prints one JSON line.  Extra keys: "plain" (the one-pair-per-workgroup
 kernel on the same workload), "phases", "mist", "roofline" and the CPU
@@ -135,8 +139,8 @@ def roofline_of(pa, method, bins, updates, kernel_ms, workload, D, P, frames):
     return r
 
 
-def emit(args, value, ms_step, config, extra):
-    out = {"metric": METRIC, "value": value, "unit": "pair-frame updates/s", "n_gpus": 1, "steps": args.steps,
+def emit(args, value, ms_step, config, extra, metric=METRIC, unit="pair-frame updates/s"):
+    out = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "u8", "data": "synthetic", "config": config}
     out.update(extra)
@@ -342,13 +346,59 @@ def run_batch(args, pa, nthreads):
           "mist_totals": mist, "roofline": roofline, "cpu_baseline": cpu, "tuning": tune})
 
 
+def run_readin(args, pa, nthreads):
+    atoms, frames = 1500, 100_000
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "protein.bat")
+        t0 = time.time()
+        pa.synth_write_bat(path, atoms, frames, 2, nthreads)
+        size = os.path.getsize(path)
+        log(f"wrote {path}: {size / 1e9:.2f} GB in {time.time() - t0:.1f} s")
+        eng = pa.Engine(0)
+
+        def streamed():
+            with pa.Bat(path) as b:
+                eng.load_bat(b)
+            eng.sync()
+
+        def host():
+            with pa.Bat(path) as b:
+                v = b.read_frames()
+            eng.set_trajectory(v)
+            eng.sync()
+        for _ in range(args.warmup):
+            streamed()
+            host()
+        st_ms, host_ms = [], []
+        for _ in range(args.steps):
+            t = time.perf_counter()
+            streamed()
+            st_ms.append(1e3 * (time.perf_counter() - t))
+            t = time.perf_counter()
+            host()
+            host_ms.append(1e3 * (time.perf_counter() - t))
+            log(f"read-in: streamed {st_ms[-1]:.1f} ms, host path {host_ms[-1]:.1f} ms")
+        # the streamed values feed the same prepare as the host-path values
+        eng.prepare([50] * 6)
+        eng.close()
+    ms = sum(st_ms) / len(st_ms)
+    hms = sum(host_ms) / len(host_ms)
+    emit(args, size / (ms / 1e3), ms,
+         {"workload": "readin_1500x100k", "atoms": atoms, "dofs": 3 * atoms - 6, "frames": frames,
+          "file_bytes": size, "precision": "fp32"},
+         {"host_path": {"ms": hms, "bytes_per_s": size / (hms / 1e3),
+                        "what": "parent_bat_read_frames (host de-interleave) + set_trajectory (pageable H2D)"},
+          "speedup_vs_host_path": hms / ms, "roofline": None, "cpu_baseline": None},
+         metric=".bat read-in (file -> frame-major values in HBM), bytes/s", unit="B/s")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="protein_1500x100k_b50",
-                    choices=sorted(WORKLOADS) + ["sweep_1500x100k", "batch64_b30"])
+                    choices=sorted(WORKLOADS) + ["sweep_1500x100k", "batch64_b30", "readin_1500x100k"])
     ap.add_argument("--method", default="auto", choices=["auto", "plain", "atomic", "mfma"])
     ap.add_argument("--plain-steps", type=int, default=1, help="timed steps of the plain kernel (0: skip)")
     ap.add_argument("--cpu-sample-atoms", type=int, default=500)
@@ -367,6 +417,8 @@ def main():
         run_sweep(args, pa, nthreads)
     elif args.workload == "batch64_b30":
         run_batch(args, pa, nthreads)
+    elif args.workload == "readin_1500x100k":
+        run_readin(args, pa, nthreads)
     else:
         run_single(args, pa, nthreads)
 

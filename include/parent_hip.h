@@ -83,6 +83,17 @@ int parent_bat_info(const parent_bat* bat, int* n_dihedrals, int* n_frames, int*
  * PARENT.cpp:443-472).  Fails with PARENT_E_IO on a truncated file. */
 int parent_bat_read_frames(parent_bat* bat, void* values);
 
+/* Bytes of one frame record: 11 floats (time, xtcPrec, box), then 6
+ * external coordinates and 3 nD + 3 values in the file's precision
+ * (io_binary.cpp:281-368). */
+int64_t parent_bat_record_bytes(const parent_bat* bat);
+
+/* Copies frame records [first_frame, first_frame + n_frames) verbatim into
+ * `records` (n_frames * parent_bat_record_bytes bytes; pinned memory lets the
+ * caller DMA it straight to the device).  The streaming half of
+ * read_BAT_frame (io_binary.cpp:281-368): the values are not reordered. */
+int parent_bat_read_raw(parent_bat* bat, int first_frame, int n_frames, void* records);
+
 void parent_bat_close(parent_bat* bat);
 
 /* Writes a .par v4: header copied from the .bat's (write_PAR_header,
@@ -153,6 +164,16 @@ const char* parent_engine_error(const parent_engine* eng);
  * device now; on_device == 1: device pointer the engine borrows until the
  * next set_trajectory/destroy.  Replaces the read-in + scatter of
  * PARENT.cpp:443-493 (the data then lives in HBM, not per-rank RAM). */
+/* Streams the frames of an opened .bat into the engine: the file is read in
+ * ~32 MB chunks of raw frame records into pinned buffers, each chunk is
+ * DMA'd to the device while the next one is read, and a kernel reorders the
+ * records into frame-major values in HBM.  Same result as
+ * parent_bat_read_frames + parent_engine_set_trajectory(on_device = 0),
+ * without the host copy of the trajectory.  Replaces the read-in loop of
+ * PARENT.cpp:443-493 (read_BAT_frame, io_binary.cpp:281-368).
+ * parent_stats.ms_upload reports the whole load (file read included). */
+int parent_engine_load_bat(parent_engine* eng, parent_bat* bat);
+
 int parent_engine_set_trajectory(parent_engine* eng, int n_dihedrals, int n_frames,
                                  int double_prec, const void* values, int on_device);
 

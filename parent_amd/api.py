@@ -78,6 +78,9 @@ def lib() -> C.CDLL:
         "parent_synth_values": (i32, [i32, i32, u64, p, i32]),
         "parent_synth_write_bat": (i32, [C.c_char_p, i32, i32, u64, i32]),
         "parent_mie_write": (i32, [C.c_char_p, i32, dp, dp, i32, dp]),
+        "parent_bat_record_bytes": (i64, [p]),
+        "parent_bat_read_raw": (i32, [p, i32, i32, p]),
+        "parent_engine_load_bat": (i32, [p, p]),
     }
     del i64
     for name, (res, args) in sig.items():
@@ -163,6 +166,17 @@ class Bat:
         """Frame-major values [N][D] (float32 or float64)."""
         out = np.empty((self.n_frames, self.D), dtype=np.float64 if self.double_prec else np.float32)
         _check_host(lib().parent_bat_read_frames(self._h, _ptr(out)))
+        return out
+
+    @property
+    def record_bytes(self) -> int:
+        """Bytes of one frame record (io_binary.cpp:281-368)."""
+        return int(lib().parent_bat_record_bytes(self._h))
+
+    def read_raw(self, first: int, n: int) -> np.ndarray:
+        """Frame records [first, first + n) as stored in the file, [n][record_bytes] uint8."""
+        out = np.empty((n, self.record_bytes), dtype=np.uint8)
+        _check_host(lib().parent_bat_read_raw(self._h, first, n, _ptr(out)))
         return out
 
     def write_par(self, path: str, bins, s1d: np.ndarray, s2d: np.ndarray) -> None:
@@ -263,6 +277,11 @@ class Engine:
         self._check(lib().parent_engine_set_trajectory(self._h, nD, v.shape[0], int(v.dtype == np.float64),
                                                        _ptr(v), 0))
         self.n_dihedrals, self.n_frames = nD, v.shape[0]
+
+    def load_bat(self, bat: "Bat") -> None:
+        """Streams an opened .bat's frames to the device (parent_engine_load_bat)."""
+        self._check(lib().parent_engine_load_bat(self._h, bat._h))
+        self.n_dihedrals, self.n_frames = bat.n_dihedrals, bat.n_frames
 
     def prepare(self, bins) -> None:
         b = np.ascontiguousarray(bins, dtype=np.int32)
@@ -462,7 +481,7 @@ class Batch:
 def run_parent(bat_path: str, par_path: str, bins, method: int = METHOD_AUTO, device: int = 0) -> dict:
     """PARENT.x in.bat out.par b1D a1D d1D b2D a2D d2D (PARENT.cpp:93-1512) on the GPU."""
     with Bat(bat_path) as bat, Engine(device) as eng:
-        eng.set_trajectory(bat.read_frames())
+        eng.load_bat(bat)
         eng.prepare(bins)
         eng.pairs(method)
         bat.write_par(par_path, bins, eng.s1d(), eng.s2d())

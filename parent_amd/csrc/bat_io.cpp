@@ -8,6 +8,9 @@
 #include "../../include/parent_hip.h"
 #include "host_common.h"
 
+#include <unistd.h>
+
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -184,7 +187,7 @@ int parent_bat_read_frames(parent_bat* b, void* values) {
     const size_t D = (size_t)parent::n_dofs(nD);
     const size_t vsz = b->double_prec ? 8 : 4;
     const size_t rec = 11 * 4 + (6 + 3 + 3 * (size_t)nD) * vsz;
-    if (fseek(b->f, b->frames_offset, SEEK_SET) != 0) {
+    if (fseeko(b->f, (off_t)b->frames_offset, SEEK_SET) != 0) {
         set_last_error("seek failed");
         return PARENT_E_IO;
     }
@@ -202,6 +205,46 @@ int parent_bat_read_frames(parent_bat* b, void* values) {
             else
                 deinterleave<float>(buf.data() + k * rec, nD, (float*)values + (f0 + k) * D);
         }
+    }
+    return PARENT_OK;
+}
+
+int64_t parent_bat_record_bytes(const parent_bat* b) {
+    if (!b) return -1;
+    const int64_t vsz = b->double_prec ? 8 : 4;
+    return 11 * 4 + (6 + 3 + 3 * (int64_t)b->nD) * vsz;
+}
+
+int parent_bat_read_raw(parent_bat* b, int first_frame, int n_frames, void* records) {
+    if (!b || !records || first_frame < 0 || n_frames < 0 || (int64_t)first_frame + n_frames > b->N) {
+        set_last_error("parent_bat_read_raw: invalid argument");
+        return PARENT_E_INVALID;
+    }
+    const int64_t rec = parent_bat_record_bytes(b);
+    const int64_t at = b->frames_offset + (int64_t)first_frame * rec;
+    const int64_t bytes = (int64_t)n_frames * rec;
+    // positioned reads, split over up to 8 threads: a single reader is bound
+    // by one core's copy out of the page cache
+    const int fd = fileno(b->f);
+    const int64_t kPart = 4 << 20;
+    const int parts = (int)std::min<int64_t>(8, std::max<int64_t>(1, bytes / kPart));
+    int bad = 0;
+#pragma omp parallel for num_threads(parts) schedule(static, 1) reduction(| : bad)
+    for (int k = 0; k < parts; k++) {
+        const int64_t lo = bytes * k / parts, hi = bytes * (k + 1) / parts;
+        char* dst = static_cast<char*>(records);
+        for (int64_t off = lo; off < hi;) {
+            const ssize_t got = pread(fd, dst + off, (size_t)(hi - off), (off_t)(at + off));
+            if (got <= 0) {
+                bad = 1;
+                break;
+            }
+            off += got;
+        }
+    }
+    if (bad) {
+        set_last_error("An ERROR has occurred while reading the frames (file truncated).");
+        return PARENT_E_IO;
     }
     return PARENT_OK;
 }

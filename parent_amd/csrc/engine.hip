@@ -16,6 +16,7 @@
 #include "kernels.h"
 
 #include <algorithm>
+#include <chrono>
 #include <climits>
 #include <cmath>
 #include <cstring>
@@ -758,21 +759,18 @@ void parent_engine_destroy(parent_engine* e) {
 
 const char* parent_engine_error(const parent_engine* e) { return e ? e->err.c_str() : "null engine"; }
 
-int parent_engine_set_trajectory(parent_engine* e, int n_dihedrals, int n_frames, int double_prec,
-                                 const void* values, int on_device) {
-    if (!e || !values) return PARENT_E_INVALID;
-    if (n_dihedrals < 1 || n_frames < 1 || (double_prec != 0 && double_prec != 1))
-        return fail(e, PARENT_E_INVALID, "need n_dihedrals >= 1, n_frames >= 1, double_prec in {0,1}");
-    HIPCHK(e, hipSetDevice(e->device));
-    HIPCHK(e, hipStreamSynchronize(e->stream));
+namespace {
+
+// Forget the previous trajectory and every result computed from it.
+Traj& reset_trajectory(parent_engine* e, int nD, int N, int dp) {
     drop_graph(e);
     e->traj_serial++;
     Traj& t = e->traj;
-    t.nD = n_dihedrals;
-    t.N = n_frames;
-    t.dp = double_prec;
-    t.D = n_dofs(n_dihedrals);
-    t.Np = (int)(((long long)n_frames + kFrameAlign - 1) / kFrameAlign * kFrameAlign);
+    t.nD = nD;
+    t.N = N;
+    t.dp = dp;
+    t.D = n_dofs(nD);
+    t.Np = (int)(((long long)N + kFrameAlign - 1) / kFrameAlign * kFrameAlign);
     t.values = nullptr;
     e->sets.resize(1);
     e->active = 0;
@@ -780,6 +778,73 @@ int parent_engine_set_trajectory(parent_engine* e, int n_dihedrals, int n_frames
     e->sets[0]->s2d_valid = false;
     e->sets[0]->ent_nD = -1;
     e->stats = parent_stats{};
+    return t;
+}
+
+// Host side of the streamed .bat load: two pinned record buffers, two events.
+// The destructor waits for the stream first, so no DMA still reads a buffer
+// when it is freed (also on error paths).
+struct BatStream {
+    hipStream_t stream;
+    void* pinned[2] = {nullptr, nullptr};
+    hipEvent_t copied[2] = {nullptr, nullptr};
+    explicit BatStream(hipStream_t s) : stream(s) {}
+    ~BatStream() {
+        (void)hipStreamSynchronize(stream);
+        for (int b = 0; b < 2; b++) {
+            if (pinned[b]) (void)hipHostFree(pinned[b]);
+            if (copied[b]) (void)hipEventDestroy(copied[b]);
+        }
+    }
+};
+
+}  // namespace
+
+int parent_engine_load_bat(parent_engine* e, parent_bat* bat) {
+    if (!e || !bat) return PARENT_E_INVALID;
+    int nD = 0, N = 0, dp = 0;
+    if (parent_bat_info(bat, &nD, &N, &dp) != PARENT_OK) return PARENT_E_INVALID;
+    if (nD < 1) return fail(e, PARENT_E_INVALID, "need n_dihedrals >= 1");
+    const long long rec = parent_bat_record_bytes(bat);
+    const auto t0 = std::chrono::steady_clock::now();
+    HIPCHK(e, hipSetDevice(e->device));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    Traj& t = reset_trajectory(e, nD, N, dp);
+    const size_t vsz = dp ? 8 : 4;
+    HIPCHK(e, t.values_own.ensure((size_t)N * t.D * vsz));
+    // chunks of ~32 MB of records: the file read of chunk k+1 overlaps the
+    // DMA and the de-interleave kernel of chunk k
+    const int chunk = (int)std::max(1LL, std::min<long long>(N, (32LL << 20) / rec));
+    DevBuf stage[2];
+    BatStream hs(e->stream);
+    for (int b = 0; b < 2; b++) {
+        HIPCHK(e, stage[b].ensure((size_t)chunk * rec));
+        HIPCHK(e, hipHostMalloc(&hs.pinned[b], (size_t)chunk * rec, hipHostMallocDefault));
+        HIPCHK(e, hipEventCreateWithFlags(&hs.copied[b], hipEventDisableTiming));
+    }
+    char* out = static_cast<char*>(t.values_own.p);
+    for (int f0 = 0, k = 0; f0 < N; f0 += chunk, k++) {
+        const int b = k & 1, nf = std::min(chunk, N - f0);
+        if (k >= 2) HIPCHK(e, hipEventSynchronize(hs.copied[b]));
+        if (int rc = parent_bat_read_raw(bat, f0, nf, hs.pinned[b])) return fail(e, rc, parent_last_error());
+        HIPCHK(e, hipMemcpyAsync(stage[b].p, hs.pinned[b], (size_t)nf * rec, hipMemcpyHostToDevice, e->stream));
+        HIPCHK(e, hipEventRecord(hs.copied[b], e->stream));
+        HIPCHK(e, launch_bat_records(stage[b].p, rec, nf, nD, dp, out + (size_t)f0 * t.D * vsz, e->stream));
+    }
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    t.values = t.values_own.p;
+    e->stats.ms_upload = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return PARENT_OK;
+}
+
+int parent_engine_set_trajectory(parent_engine* e, int n_dihedrals, int n_frames, int double_prec,
+                                 const void* values, int on_device) {
+    if (!e || !values) return PARENT_E_INVALID;
+    if (n_dihedrals < 1 || n_frames < 1 || (double_prec != 0 && double_prec != 1))
+        return fail(e, PARENT_E_INVALID, "need n_dihedrals >= 1, n_frames >= 1, double_prec in {0,1}");
+    HIPCHK(e, hipSetDevice(e->device));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    Traj& t = reset_trajectory(e, n_dihedrals, n_frames, double_prec);
     const size_t bytes = (size_t)t.N * t.D * (double_prec ? 8 : 4);
     if (on_device) {
         t.values_own.release();

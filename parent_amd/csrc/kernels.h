@@ -24,6 +24,10 @@ struct PrepareArgs {
 hipError_t launch_prepare(const PrepareArgs& a, hipStream_t s);
 // n ln n for n = 0..N (FP64 table of the fused epilogues)
 hipError_t launch_nlogn(int N, double* nlogn, hipStream_t s);
+// raw .bat frame records (n_frames x rec_bytes, 4-byte aligned) -> frame-major
+// values (float, or double when dp)
+hipError_t launch_bat_records(const void* records, long long rec_bytes, int n_frames, int nD, int dp,
+                              void* out, hipStream_t s);
 
 // Common inputs of every pair kernel.
 struct PairInputs {

@@ -91,13 +91,6 @@ int main(int argc, char* argv[]) {
         return 1;
     }
     const size_t D = 3 * (size_t)nD + 3;
-    std::vector<char> values(D * (size_t)N * (dp ? 8 : 4));
-    if ((rc = parent_bat_read_frames(bat, values.data())) != PARENT_OK) {
-        std::cerr << parent_last_error() << std::endl;
-        std::cerr << "An ERROR has occurred while reading the file " << argv[1] << " . Quitting Program.\n";
-        parent_bat_close(bat);
-        return 1;
-    }
     const char* dev_env = getenv("PARENT_HIP_DEVICE");
     parent_engine* eng = nullptr;
     if ((rc = parent_engine_create(dev_env ? atoi(dev_env) : 0, &eng)) != PARENT_OK) {
@@ -111,8 +104,17 @@ int main(int argc, char* argv[]) {
         parent_bat_close(bat);
         return 1;
     };
-    if (parent_engine_set_trajectory(eng, nD, N, dp, values.data(), 0) != PARENT_OK) return die("upload");
-    std::vector<char>().swap(values);
+    // frames stream from the file through pinned buffers straight into HBM
+    if ((rc = parent_engine_load_bat(eng, bat)) != PARENT_OK) {
+        if (rc == PARENT_E_IO) {
+            std::cerr << parent_engine_error(eng) << std::endl;
+            std::cerr << "An ERROR has occurred while reading the file " << argv[1] << " . Quitting Program.\n";
+            parent_engine_destroy(eng);
+            parent_bat_close(bat);
+            return 1;
+        }
+        return die("read-in");
+    }
     std::cout << "READIN AND DISTRIBUTION COMPLETED." << std::endl;
     gettimeofday(&tv2, nullptr);
     if (parent_engine_prepare(eng, bins) != PARENT_OK) return die("extrema / pre-binning / 1D");

@@ -445,6 +445,46 @@ hipError_t launch_prepare(const PrepareArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
+// .bat frame records -> frame-major values.  A record (io_binary.cpp:281-368)
+// is 11 floats (time, xtcPrec, box) then, in the file's precision, 6 external
+// coordinates, bond0, bond1, angle0 and nD triples (bond[i+2], angle[i+1],
+// dihedral[i]).  Value k of a record sits at byte 44 + k * sizeof(T); output
+// dof g takes
+//   bonds      g = 0, 1 -> k = 6, 7;   g = 2 + i      -> k = 9 + 3i
+//   angles     a = 0    -> k = 8;      a = 1 + i      -> k = 10 + 3i
+//   dihedrals  d = i                   -> k = 11 + 3i
+// Records are 4-byte aligned only (44 + 8k for doubles), so doubles are
+// assembled from two 32-bit words.  One workgroup per frame, threads over
+// the frame's D values: coalesced writes, stride-3 reads.
+template <bool DP>
+__global__ __launch_bounds__(256) void k_bat_records(const uint32_t* __restrict__ rec, long long rec_words,
+                                                     int nD, void* __restrict__ out) {
+    const int f = blockIdx.x;
+    const int nB = nD + 2, nA = nD + 1, D = 3 * nD + 3;
+    const uint32_t* r = rec + (size_t)f * rec_words + 11;
+    for (int g = threadIdx.x; g < D; g += 256) {
+        int k;
+        if (g < nB) k = g < 2 ? 6 + g : 9 + 3 * (g - 2);
+        else if (g < nB + nA) k = g == nB ? 8 : 10 + 3 * (g - nB - 1);
+        else k = 11 + 3 * (g - nB - nA);
+        if (DP) {
+            const unsigned long long lo = r[2 * k], hi = r[2 * k + 1];
+            reinterpret_cast<unsigned long long*>(out)[(size_t)f * D + g] = lo | (hi << 32);
+        } else {
+            reinterpret_cast<uint32_t*>(out)[(size_t)f * D + g] = r[k];
+        }
+    }
+}
+
+hipError_t launch_bat_records(const void* records, long long rec_bytes, int n_frames, int nD, int dp,
+                              void* out, hipStream_t s) {
+    if (n_frames <= 0) return hipSuccess;
+    const uint32_t* r = reinterpret_cast<const uint32_t*>(records);
+    if (dp) k_bat_records<true><<<n_frames, 256, 0, s>>>(r, rec_bytes / 4, nD, out);
+    else k_bat_records<false><<<n_frames, 256, 0, s>>>(r, rec_bytes / 4, nD, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_nlogn(int N, double* nlogn, hipStream_t s) {
     k_nlogn<<<(N + 256) / 256, 256, 0, s>>>(N, nlogn);
     return hipGetLastError();

@@ -271,3 +271,27 @@ def test_mie_cli_errors_exit_1(tmp_path):
     bad.write_bytes(struct.pack("<10i", 4, 2, 3, 10, 50, 50, 50, 50, 50, 50))
     r = subprocess.run([MIE_EXE, str(bad)], capture_output=True, text=True)
     assert r.returncode == 1 and "ERROR READING HEADER OF FILE" in r.stderr
+
+
+def test_bat_raw_records_are_the_file_bytes(tmp_path):
+    """parent_bat_read_raw hands back the frame records verbatim: the file's
+    last N * record_bytes bytes, for any frame window, FP32 and FP64."""
+    p64 = str(tmp_path / "d.bat")
+    write_bat_fp64(p64, pa.synth_values(10, 70, 2).astype(np.float64), 7)
+    for path, rec in [(BAT, 80 + 12 * 9), (p64, 116 + 24 * 7)]:
+        data = open(path, "rb").read()
+        with pa.Bat(path) as b:
+            assert b.record_bytes == rec
+            frames = np.frombuffer(data[len(data) - b.n_frames * rec:], dtype=np.uint8).reshape(b.n_frames, rec)
+            assert np.array_equal(b.read_raw(0, b.n_frames), frames)
+            assert np.array_equal(b.read_raw(13, 29), frames[13:42])
+            assert b.read_raw(b.n_frames, 0).shape == (0, rec)
+            with pytest.raises(pa.ParentError):
+                b.read_raw(b.n_frames - 2, 3)
+    short = tmp_path / "short.bat"
+    short.write_bytes(open(BAT, "rb").read()[:-50])
+    with pa.Bat(str(short)) as b:
+        b.read_raw(0, b.n_frames - 1)
+        with pytest.raises(pa.ParentError) as ei:
+            b.read_raw(0, b.n_frames)
+        assert ei.value.code == 3
