@@ -58,4 +58,37 @@ __host__ __device__ __forceinline__ void pair_of_index(long long p, int D, int* 
     *gj = (int)(p - start(i) + i + 1);
 }
 
+// Wave-wide sums without LDS traffic: DPP lane moves inside each 16-lane row
+// (pair swap, quad swap, half-row mirror, row mirror: every lane then holds
+// its row's sum), then the four row sums are read from lanes 0, 16, 32, 48
+// and added in that order.  The result is wave-uniform and its summation
+// order fixed, so it is bit-reproducible.  Every lane of the wave must be
+// active.  (The LDS-based shuffles this replaces compete with the LDS atomics
+// of other workgroups on the CU.)
+template <int CTRL>
+__device__ __forceinline__ int dpp_mov(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_mov<CTRL>((int)(b & 0xFFFFFFFFLL)), hi = dpp_mov<CTRL>((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ __forceinline__ int lane_value(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
+__device__ __forceinline__ double lane_value(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xFFFFFFFFLL), lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+template <class T>
+__device__ __forceinline__ T wave_total(T v) {
+    v += dpp_mov<0xB1>(v);   // quad_perm [1,0,3,2]
+    v += dpp_mov<0x4E>(v);   // quad_perm [2,3,0,1]
+    v += dpp_mov<0x141>(v);  // row_half_mirror
+    v += dpp_mov<0x140>(v);  // row_mirror
+    return ((lane_value(v, 0) + lane_value(v, 16)) + lane_value(v, 32)) + lane_value(v, 48);
+}
+
 }  // namespace parent

@@ -14,6 +14,7 @@
 #include "kernels.h"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace parent {
 
@@ -24,16 +25,37 @@ __device__ __forceinline__ unsigned byte_of(const uint4& w, int e) {
     return (v >> (8 * (e & 3))) & 0xFFu;
 }
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+// (byte E of w) << amount, and 1 << (byte E of s): one v_lshlrev_b32 with a
+// byte-select (SDWA) operand each.  Written out because the compiler emits a
+// shift + mask pair for some bytes.  Both operands must be VGPRs.
+#define PARENT_SDWA_SHL(E)                                                                             \
+    if constexpr (SEL == E) {                                                                          \
+        asm("v_lshlrev_b32_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD "        \
+            "src1_sel:BYTE_" #E                                                                        \
+            : "=v"(r)                                                                                  \
+            : "v"(amount), "v"(w));                                                                    \
+    }
+template <int SEL>
+__device__ __forceinline__ unsigned byte_shl(unsigned w, unsigned amount) {
+    unsigned r;
+    PARENT_SDWA_SHL(0) PARENT_SDWA_SHL(1) PARENT_SDWA_SHL(2) PARENT_SDWA_SHL(3)
+    return r;
 }
-__device__ __forceinline__ int wave_sum(int v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+#undef PARENT_SDWA_SHL
+#define PARENT_SDWA_ONE_SHL(E)                                                                         \
+    if constexpr (SEL == E) {                                                                          \
+        asm("v_lshlrev_b32_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:BYTE_" #E      \
+            " src1_sel:DWORD"                                                                          \
+            : "=v"(r)                                                                                  \
+            : "v"(s), "v"(one));                                                                       \
+    }
+template <int SEL>
+__device__ __forceinline__ unsigned one_shl_byte(unsigned s, unsigned one) {
+    unsigned r;
+    PARENT_SDWA_ONE_SHL(0) PARENT_SDWA_ONE_SHL(1) PARENT_SDWA_ONE_SHL(2) PARENT_SDWA_ONE_SHL(3)
+    return r;
 }
+#undef PARENT_SDWA_ONE_SHL
 
 }  // namespace
 
@@ -269,8 +291,8 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_atomic(PairInputs in0, TileQueu
                 occ++;
             }
         }
-        T = wave_sum(T);
-        occ = wave_sum(occ);
+        T = wave_total(T);
+        occ = wave_total(occ);
         if (lane == 0)
             in.s2d[par_offset(nD, gi, gj)] =
                 ((in.logN + in.c2[gi]) + in.c2[gj] + (occ - 1.0) / (2.0 * N)) - T / N;
@@ -370,6 +392,9 @@ __global__ __launch_bounds__(BLOCK, 1) void k_pairs_sliced(PairInputs in0, TileQ
         // byte offset of row byte a: (a or a & ~1) * rowmul; of column byte b: b * 64
         const unsigned rowmul = (unsigned)Bc[0] * (PACK == 1 ? 64u : 32u);
         const unsigned s4 = s16 * 4u;
+        // shift amount and increment base for the byte-select shifts (VGPRs)
+        unsigned six = 6u, one = 1u;
+        asm("" : "+v"(six), "+v"(one));
         auto process = [&](const uint4* R, const uint4* C) {
 #pragma unroll
             for (int k = 0; k < 4; k++) {
@@ -379,25 +404,35 @@ __global__ __launch_bounds__(BLOCK, 1) void k_pairs_sliced(PairInputs in0, TileQ
                     const unsigned w = comp(R[r], k);
                     rw[r] = PACK == 1 ? w : (w & 0xFEFEFEFEu);
                     rsh[r] = (w & 0x01010101u) << 4;
+                    // Keep the masked words opaque: otherwise the masks are
+                    // folded back into per-byte shift/and pairs (~7 VALU per
+                    // update) instead of byte-select (SDWA) operands of the
+                    // multiply and the increment shift (~4.5 VALU).
+                    if constexpr (PACK == 2) asm("" : "+v"(rw[r]), "+v"(rsh[r]));
                 }
 #pragma unroll
                 for (int c = 0; c < TC; c++) cw[c] = comp(C[c], k);
-#pragma unroll
-                for (int e = 0; e < 4; e++) {
+                auto frame = [&](auto ec) {
+                    constexpr int e = decltype(ec)::value;
                     unsigned ro[TR], inc[TR], co[TC];
 #pragma unroll
                     for (int r = 0; r < TR; r++) {
                         ro[r] = __umul24((rw[r] >> (8 * e)) & 0xFFu, rowmul) + s4;
-                        inc[r] = PACK == 1 ? 1u : 1u << ((rsh[r] >> (8 * e)) & 0xFFu);
+                        if constexpr (PACK == 1) inc[r] = 1u;
+                        else inc[r] = one_shl_byte<e>(rsh[r], one);
                     }
 #pragma unroll
-                    for (int c = 0; c < TC; c++) co[c] = ((cw[c] >> (8 * e)) & 0xFFu) << 6;
+                    for (int c = 0; c < TC; c++) co[c] = byte_shl<e>(cw[c], six);
 #pragma unroll
                     for (int r = 0; r < TR; r++)
 #pragma unroll
                         for (int c = 0; c < TC; c++)
                             atomicAdd(reinterpret_cast<unsigned*>(hb + (r * TC + c) * kSlotB + ro[r] + co[c]), inc[r]);
-                }
+                };
+                frame(std::integral_constant<int, 0>{});
+                frame(std::integral_constant<int, 1>{});
+                frame(std::integral_constant<int, 2>{});
+                frame(std::integral_constant<int, 3>{});
             }
         };
         // two register sets: the next iteration's index words load while
@@ -520,8 +555,8 @@ __global__ __launch_bounds__(BLOCK, 1) void k_pairs_sliced(PairInputs in0, TileQ
                 occ++;
             }
         }
-        T = wave_sum(T);
-        occ = wave_sum(occ);
+        T = wave_total(T);
+        occ = wave_total(occ);
         if (lane == 0) {
             sT[wid] = T;
             sOcc[wid] = occ;

@@ -44,17 +44,6 @@ constexpr int kMaxUnits = 4;      // (dof, frame group) units per thread: 2 * TD
 // clobber keeps the compiler from moving LDS accesses across it.
 __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
-__device__ __forceinline__ double wave_sum_d(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-__device__ __forceinline__ int wave_sum_i(int v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
 }  // namespace
 
 template <bool BATCH>
@@ -211,8 +200,8 @@ __global__ __launch_bounds__(256, 2) void k_pairs_mfma(PairInputs in0, int B, in
                 occ++;
             }
         }
-        T = wave_sum_d(T);
-        occ = wave_sum_i(occ);
+        T = wave_total(T);
+        occ = wave_total(occ);
         if (lane == 0) in.s2d[par_offset(nD, gi, gj)] = ((in.logN + in.c2[gi]) + in.c2[gj] + (occ - 1.0) / (2.0 * N)) - T / N;
     }
 }
