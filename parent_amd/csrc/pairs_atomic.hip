@@ -20,6 +20,11 @@ namespace parent {
 
 namespace {
 
+// Workgroup barrier that waits for this wave's LDS operations only: global
+// loads in flight stay in flight (__syncthreads() would wait for them too).
+// The memory clobber keeps LDS accesses from moving across it.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
 __device__ __forceinline__ unsigned byte_of(const uint4& w, int e) {
     const unsigned v = e < 4 ? w.x : (e < 8 ? w.y : (e < 12 ? w.z : w.w));
     return (v >> (8 * (e & 3))) & 0xFFu;
@@ -199,9 +204,6 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_atomic(PairInputs in0, TileQueu
     for (int c = 1; c < TC; c++) uniform = uniform && Bc[c] == Bc[0];
     const bool fast = uniform && valid == (1u << (TR * TC)) - 1u;
 
-    for (int i = threadIdx.x; i < TR * TC * SLOTW; i += BLOCK) hist[i] = 0;
-    __syncthreads();
-
     const int per_iter = BLOCK * 16;
     const int niter = (N + per_iter - 1) / per_iter;
     const int nfull = N / per_iter;
@@ -211,17 +213,27 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_atomic(PairInputs in0, TileQueu
     for (int r = 0; r < TR; r++) rows[r] = in.bins + (size_t)(gr[r] < D ? gr[r] : 0) * in.Np;
 #pragma unroll
     for (int c = 0; c < TC; c++) cols[c] = in.bins + (size_t)(gc[c] < D ? gc[c] : 0) * in.Np;
-
-    int it0 = 0;
-    if (fast) {
-        const unsigned B = (unsigned)Bc[0];
-        for (int it = 0; it < nfull; it++) {
-            const int f = it * per_iter + threadIdx.x * 16;
-            uint4 R[TR], C[TC];
+    // index words of the fast fill's first iteration, requested before the
+    // clear; the barrier after it waits for LDS operations only
+    auto load = [&](uint4* R, uint4* C, int it) {
+        const int f = it * per_iter + threadIdx.x * 16;
+        if (f < in.Np) {
 #pragma unroll
             for (int r = 0; r < TR; r++) R[r] = *reinterpret_cast<const uint4*>(rows[r] + f);
 #pragma unroll
             for (int c = 0; c < TC; c++) C[c] = *reinterpret_cast<const uint4*>(cols[c] + f);
+        }
+    };
+    uint4 R0[TR], C0[TC];
+    if (fast && nfull > 0) load(R0, C0, 0);
+
+    for (int i = threadIdx.x; i < TR * TC * SLOTW; i += BLOCK) hist[i] = 0;
+    lds_barrier();
+
+    int it0 = 0;
+    if (fast && nfull > 0) {
+        const unsigned B = (unsigned)Bc[0];
+        auto process = [&](const uint4* R, const uint4* C) {
 #pragma unroll
             for (int e = 0; e < 16; e++) {
                 unsigned ro[TR];
@@ -232,8 +244,22 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_atomic(PairInputs in0, TileQueu
 #pragma unroll
                     for (int c = 0; c < TC; c++) atomicAdd(&hist[(r * TC + c) * SLOTW + ro[r] + byte_of(C[c], e)], 1u);
             }
+        };
+        // two register sets: the next iteration's words load while this
+        // iteration's atomics issue
+        uint4 R1[TR], C1[TC];
+        int it = 0;
+        for (; it + 2 <= nfull; it += 2) {
+            load(R1, C1, it + 1);
+            process(R0, C0);
+            load(R0, C0, it + 2);
+            process(R1, C1);
         }
-        it0 = nfull;
+        if (it < nfull) {
+            process(R0, C0);
+            it++;
+        }
+        it0 = it;
     }
     // general fill: diagonal-band / edge tiles, mixed bin counts, and the
     // final partial iteration
@@ -359,12 +385,6 @@ __global__ __launch_bounds__(BLOCK, 1) void k_pairs_sliced(PairInputs in0, TileQ
     for (int c = 1; c < TC; c++) uniform = uniform && Bc[c] == Bc[0];
     const bool fast = uniform && valid == (1u << (TR * TC)) - 1u;
 
-    {
-        uint4* h4 = reinterpret_cast<uint4*>(hist);
-        for (int i = threadIdx.x; i < TR * TC * kSlotW / 4; i += BLOCK) h4[i] = make_uint4(0, 0, 0, 0);
-    }
-    __syncthreads();
-
     const int per_iter = BLOCK * 16;
     const int niter = (N + per_iter - 1) / per_iter;
     const int nfull = N / per_iter;
@@ -374,6 +394,23 @@ __global__ __launch_bounds__(BLOCK, 1) void k_pairs_sliced(PairInputs in0, TileQ
     for (int r = 0; r < TR; r++) rows[r] = in.bins + (size_t)(gr[r] < D ? gr[r] : 0) * in.Np;
 #pragma unroll
     for (int c = 0; c < TC; c++) cols[c] = in.bins + (size_t)(gc[c] < D ? gc[c] : 0) * in.Np;
+    // The fast fill's first index words are requested before the histogram
+    // clear, and the barrier after the clear waits for LDS operations only,
+    // so the load latency of a new tile overlaps the clear.
+    uint4 R0[TR], C0[TC];
+    if (fast && nfull > 0) {
+        const int f = threadIdx.x * 16;
+#pragma unroll
+        for (int r = 0; r < TR; r++) R0[r] = *reinterpret_cast<const uint4*>(rows[r] + f);
+#pragma unroll
+        for (int c = 0; c < TC; c++) C0[c] = *reinterpret_cast<const uint4*>(cols[c] + f);
+    }
+
+    {
+        uint4* h4 = reinterpret_cast<uint4*>(hist);
+        for (int i = threadIdx.x; i < TR * TC * kSlotW / 4; i += BLOCK) h4[i] = make_uint4(0, 0, 0, 0);
+    }
+    lds_barrier();
 
     // word of cell (a, b) in a slot with Bj columns, and the lane's increment
     auto cell_word = [&](unsigned a, unsigned b, unsigned Bj) {
@@ -446,8 +483,7 @@ __global__ __launch_bounds__(BLOCK, 1) void k_pairs_sliced(PairInputs in0, TileQ
                 for (int c = 0; c < TC; c++) C[c] = *reinterpret_cast<const uint4*>(cols[c] + f);
             }
         };
-        uint4 R0[TR], C0[TC], R1[TR], C1[TC];
-        load(R0, C0, 0);
+        uint4 R1[TR], C1[TC];  // R0 / C0 hold iteration 0 already
         int it = 0;
         for (; it + 2 <= nfull; it += 2) {
             load(R1, C1, it + 1);
