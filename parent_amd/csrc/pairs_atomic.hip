@@ -62,6 +62,21 @@ __device__ __forceinline__ unsigned one_shl_byte(unsigned s, unsigned one) {
 }
 #undef PARENT_SDWA_ONE_SHL
 
+// low halves a.lo + b.lo, high halves a.hi + b.hi: one v_add_u32 with
+// halfword-select (SDWA) operands each
+__device__ __forceinline__ unsigned add_lo16(unsigned a, unsigned b) {
+    unsigned r;
+    asm("v_add_u32_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:WORD_0 src1_sel:WORD_0"
+        : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+__device__ __forceinline__ unsigned add_hi16(unsigned a, unsigned b) {
+    unsigned r;
+    asm("v_add_u32_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:WORD_1 src1_sel:WORD_1"
+        : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -568,17 +583,23 @@ __global__ __launch_bounds__(BLOCK, 1) void k_pairs_sliced(PairInputs in0, TileQ
         // its group at step s, so the 16 lanes served together hit 16
         // distinct banks.
         const int ngroups = (PACK == 1 ? Bi : (Bi + 1) / 2) * Bj;
-        const unsigned* hs = hist + sl * kSlotW;
+        const uint4* hs4 = reinterpret_cast<const uint4*>(hist + sl * kSlotW);
+        // a group is 4 uint4; thread t reads them in the order rotated by
+        // (t / 2) % 4, so the 8 lanes a 128-byte read cycle serves (4 banks
+        // each, groups of alternating parity) cover all 32 banks
+        int quad[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) quad[k] = (k + (threadIdx.x >> 1)) & 3;
         for (int g = threadIdx.x; g < ngroups; g += BLOCK) {
             unsigned lo = 0, hi = 0;
 #pragma unroll
-            for (int k = 0; k < 16; k++) {
-                const unsigned w = hs[g * 16 + ((k + threadIdx.x) & 15)];
+            for (int k = 0; k < 4; k++) {
+                const uint4 v = hs4[g * 4 + quad[k]];
                 if constexpr (PACK == 1) {
-                    lo += w;
+                    lo += (v.x + v.y) + (v.z + v.w);
                 } else {
-                    lo += w & 0xFFFFu;
-                    hi += w >> 16;
+                    lo += add_lo16(v.x, v.y) + add_lo16(v.z, v.w);
+                    hi += add_hi16(v.x, v.y) + add_hi16(v.z, v.w);
                 }
             }
             if (lo) {
