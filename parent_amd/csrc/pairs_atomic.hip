@@ -325,12 +325,13 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_atomic(PairInputs in0, TileQueu
         const unsigned* h = hist + sl * SLOTW;
         double T = 0.0;
         int occ = 0;
+        // branch-free (n ln n of 0 is 0) so the unrolled table loads are in
+        // flight together; same sums, in the same order, as skipping n = 0
+#pragma unroll 4
         for (int i = lane; i < cells; i += 64) {
             const unsigned n = h[i];
-            if (n) {
-                T += in.nlogn[n];
-                occ++;
-            }
+            T += in.nlogn[n];
+            occ += n != 0;
         }
         T = wave_total(T);
         occ = wave_total(occ);
@@ -602,14 +603,15 @@ __global__ __launch_bounds__(BLOCK, 1) void k_pairs_sliced(PairInputs in0, TileQ
                     hi += add_hi16(v.x, v.y) + add_hi16(v.z, v.w);
                 }
             }
-            if (lo) {
-                T += in.nlogn[lo];
-                occ++;
-            }
-            // the odd row of a packed pair exists unless Bi is odd and this is the last pair row
-            if (PACK == 2 && hi && 2 * (g / Bj) + 1 < Bi) {
-                T += in.nlogn[hi];
-                occ++;
+            // branch-free: n ln n of 0 is 0, so both table loads issue together
+            T += in.nlogn[lo];
+            occ += lo != 0;
+            if constexpr (PACK == 2) {
+                // the odd row of a packed pair exists unless Bi is odd and
+                // this is the last pair row (whose high halves stay 0)
+                const unsigned h2 = 2 * (g / Bj) + 1 < Bi ? hi : 0u;
+                T += in.nlogn[h2];
+                occ += h2 != 0;
             }
         }
         T = wave_total(T);

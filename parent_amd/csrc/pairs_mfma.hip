@@ -193,12 +193,12 @@ __global__ __launch_bounds__(256, 2) void k_pairs_mfma(PairInputs in0, int B, in
         if (!(gi < gj && gj < D)) continue;
         double T = 0.0;
         int occ = 0;
+        // branch-free (n ln n of 0 is 0): the unrolled table loads overlap
+#pragma unroll 4
         for (int c = lane; c < B * B; c += 64) {
             const int n = cnt[(di * B + c / B) * kTile + dj * B + c % B];
-            if (n) {
-                T += in.nlogn[n];
-                occ++;
-            }
+            T += in.nlogn[n];
+            occ += n != 0;
         }
         T = wave_total(T);
         occ = wave_total(occ);
