@@ -246,7 +246,8 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_atomic(PairInputs in0, TileQueu
     lds_barrier();
 
     int it0 = 0;
-    if (fast && nfull > 0) {
+    bool tail_fast = false;
+    if (fast) {
         const unsigned B = (unsigned)Bc[0];
         auto process = [&](const uint4* R, const uint4* C) {
 #pragma unroll
@@ -275,10 +276,19 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_atomic(PairInputs in0, TileQueu
             it++;
         }
         it0 = it;
+        // the last, partial iteration: a thread whose 16 frames are all
+        // valid stays on the fast path; only the thread with the final
+        // partial chunk goes through the general loop below
+        if (it0 < niter && it0 * per_iter + (int)threadIdx.x * 16 + 16 <= N) {
+            load(R1, C1, it0);
+            process(R1, C1);
+            tail_fast = true;
+        }
     }
     // general fill: diagonal-band / edge tiles, mixed bin counts, and the
     // final partial iteration
     for (int it = it0; it < niter; it++) {
+        if (tail_fast) break;  // this thread's last iteration ran on the fast path
         const int f = it * per_iter + threadIdx.x * 16;
         uint4 R[TR], C[TC];
 #pragma unroll
@@ -441,7 +451,8 @@ __global__ __launch_bounds__(BLOCK, 1) void k_pairs_sliced(PairInputs in0, TileQ
     constexpr int kSlotB = kSlotW * 4;
     auto comp = [](const uint4& v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : (k == 2 ? v.z : v.w)); };
     int it0 = 0;
-    if (fast && nfull > 0) {
+    bool tail_fast = false;
+    if (fast) {
         // byte offset of row byte a: (a or a & ~1) * rowmul; of column byte b: b * 64
         const unsigned rowmul = (unsigned)Bc[0] * (PACK == 1 ? 64u : 32u);
         const unsigned s4 = s16 * 4u;
@@ -512,8 +523,17 @@ __global__ __launch_bounds__(BLOCK, 1) void k_pairs_sliced(PairInputs in0, TileQ
             it++;
         }
         it0 = it;
+        // the last, partial iteration: a thread whose 16 frames are all
+        // valid stays on the fast path; only the thread with the final
+        // partial chunk goes through the general loop below
+        if (it0 < niter && it0 * per_iter + (int)threadIdx.x * 16 + 16 <= N) {
+            load(R1, C1, it0);
+            process(R1, C1);
+            tail_fast = true;
+        }
     }
     for (int it = it0; it < niter; it++) {
+        if (tail_fast) break;  // this thread's last iteration ran on the fast path
         const int f = it * per_iter + threadIdx.x * 16;
         uint4 R[TR], C[TC];
 #pragma unroll
