@@ -121,6 +121,17 @@ int parent_par_read(const char* path, int* n_dihedrals, int* n_frames, int bins[
 int parent_mie_write(const char* path, int n_dihedrals, const double* s1d, const double* s2d,
                      int nthreads, double* totals);
 
+/* Builds a .bat from a GROMACS topology and a compressed (.xtc) trajectory:
+ * the .xtc -> .bat direction of the reference's BAT_builder.x (bat.cpp:82-176;
+ * topology.cpp, BAT_topology.cpp, BAT_trajectory.cpp:77-299, io_binary.cpp:41-278),
+ * with the same dihedral tree, phase angles and bytes.
+ *   backbone_names  blank-separated atom names for the phase-angle rule (-bb)
+ *   double_prec     1: FP64 values (the reference's default), 0: FP32
+ *   nthreads        threads that decode and convert frames (<= 0: OpenMP default)
+ *   n_frames        NULL or the number of frames written */
+int parent_bat_build(const char* top_path, const char* xtc_path, const char* bat_path,
+                     const char* backbone_names, int double_prec, int nthreads, int* n_frames);
+
 /* Message of the last failed parent_bat_* / parent_par_* / parent_synth_* /
  * parent_mie_* call on this thread. */
 const char* parent_last_error(void);

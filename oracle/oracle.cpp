@@ -10,6 +10,7 @@
 #include <cstring>
 #include <fstream>
 #include <string>
+#include <array>
 #include <vector>
 #include <omp.h>
 
@@ -445,23 +446,42 @@ int oracle_mist(int nD, const double* s1d, const double* s2d, int32_t* edges, do
         return lo == 0 ? hi : (lo == 1 ? 2 + hi : 5);  // BB BA BD AA AD DD
     };
     int step = 0;
+    // The argmax over In x Out scans (t1, t2, i, j) lexicographically and keeps
+    // the first maximum (strict >).  The scan is split into contiguous runs of
+    // (t1, t2, i) rows, one per thread; each run keeps its own first maximum
+    // and the runs are merged in order with strict >, which selects the same
+    // element as the sequential scan.
+    struct Best {
+        double v = -1e200;
+        int t1 = -1, t2 = -1, i = -1, j = -1;
+    };
+    std::vector<std::array<int, 3>> rows;
     while (Out[0].size() + Out[1].size() + Out[2].size() > 0) {
-        double best = -1e200;
-        int bt1 = -1, bt2 = -1, bi = -1, bj = -1;
+        rows.clear();
         for (int t1 = 0; t1 < 3; t1++)
             for (int t2 = 0; t2 < 3; t2++)
-                for (size_t i = 0; i < In[t1].size(); i++)
-                    for (size_t j = 0; j < Out[t2].size(); j++) {
-                        double v = oracle_mutual(nD, s1d, s2d, first_of_type(nD, t1) + In[t1][i],
-                                                 first_of_type(nD, t2) + Out[t2][j]);
-                        if (v > best) {
-                            best = v;
-                            bt1 = t1;
-                            bt2 = t2;
-                            bi = (int)i;
-                            bj = (int)j;
-                        }
-                    }
+                for (size_t i = 0; i < In[t1].size(); i++) rows.push_back({t1, t2, (int)i});
+        std::vector<Best> part(omp_get_max_threads());
+#pragma omp parallel num_threads((int)part.size())
+        {
+            const int me = omp_get_thread_num(), nth = omp_get_num_threads();
+            const size_t r0 = rows.size() * me / nth, r1 = rows.size() * (me + 1) / nth;
+            Best b;
+            for (size_t r = r0; r < r1; r++) {
+                const int t1 = rows[r][0], t2 = rows[r][1], i = rows[r][2];
+                for (size_t j = 0; j < Out[t2].size(); j++) {
+                    const double v = oracle_mutual(nD, s1d, s2d, first_of_type(nD, t1) + In[t1][i],
+                                                   first_of_type(nD, t2) + Out[t2][j]);
+                    if (v > b.v) b = {v, t1, t2, i, (int)j};
+                }
+            }
+            part[me] = b;
+        }
+        Best w;
+        for (const Best& b : part)
+            if (b.v > w.v) w = b;
+        const double best = w.v;
+        const int bt1 = w.t1, bt2 = w.t2, bi = w.i, bj = w.j;
         if (bt1 < 0) return 1;
         totalMutual += best;
         edges[4 * step + 0] = bt1;

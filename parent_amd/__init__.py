@@ -20,6 +20,7 @@ from .api import (  # noqa: F401
     METHOD_PLAIN,
     Bat,
     Batch,
+    bat_build,
     Engine,
     ParentError,
     dof_counts,

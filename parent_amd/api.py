@@ -81,6 +81,7 @@ def lib() -> C.CDLL:
         "parent_bat_record_bytes": (i64, [p]),
         "parent_bat_read_raw": (i32, [p, i32, i32, p]),
         "parent_engine_load_bat": (i32, [p, p]),
+        "parent_bat_build": (i32, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, i32, i32, C.POINTER(i32)]),
     }
     del i64
     for name, (res, args) in sig.items():
@@ -229,6 +230,16 @@ def mie_write(path, n_dihedrals: int, s1d: np.ndarray, s2d: np.ndarray, nthreads
     _check_host(lib().parent_mie_write(os.fsencode(path) if path is not None else None, n_dihedrals,
                                        _ptr(s1), _ptr(s2), nthreads, _ptr(tot)))
     return tot
+
+
+def bat_build(top: str, xtc: str, bat: str, backbone: str = "CA C N H1 O1", double_prec: int = 1,
+              nthreads: int = 0) -> int:
+    """BAT_builder.x -t top -x xtc -o bat -bb backbone (bat.cpp:82-176): writes the .bat and
+    returns its frame count."""
+    n = C.c_int()
+    _check_host(lib().parent_bat_build(os.fsencode(top), os.fsencode(xtc), os.fsencode(bat), backbone.encode(),
+                                       double_prec, nthreads, C.byref(n)))
+    return n.value
 
 
 def synth_values(atoms: int, n_frames: int, seed: int, nthreads: int = 0) -> np.ndarray:

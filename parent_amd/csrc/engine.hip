@@ -759,6 +759,8 @@ void parent_engine_destroy(parent_engine* e) {
 
 const char* parent_engine_error(const parent_engine* e) { return e ? e->err.c_str() : "null engine"; }
 
+}  // extern "C"
+
 namespace {
 
 // Forget the previous trajectory and every result computed from it.
@@ -799,6 +801,8 @@ struct BatStream {
 };
 
 }  // namespace
+
+extern "C" {
 
 int parent_engine_load_bat(parent_engine* e, parent_bat* bat) {
     if (!e || !bat) return PARENT_E_INVALID;

@@ -12,9 +12,15 @@ sources by `make -C oracle ref` (oracle/_ref/):
   chain12_mixed_MIST.txt      get_PAR_MIST.x chain12_mixed.par
   ubq_sample_MIE_tail.txt     test_system/sample_output/sample_output_MIE.txt (reference data)
   ubq_sample_MIST_tail.txt    test_system/sample_output/sample_output_MIST.txt (reference data)
+  ubq/UBQ_UBM2.top, .xtc      test_system/UBQ_UBM2.top / .xtc (reference data: the inputs of its
+                              end-to-end test)
+  ubq/bat_sha256.txt          SHA-256 of BAT_builder.x -t UBQ_UBM2.top -x UBQ_UBM2.xtc -o UBQ.bat
+                              -bb "CA C N H1 O1" (FP64), of the same with --single_precision, and
+                              with -bb "CA N"
 
-Usage: python tests/golden/make_golden.py
+Usage: python tests/golden/make_golden.py [--ubq-only]
 """
+import hashlib
 import os
 import shutil
 import subprocess
@@ -38,7 +44,28 @@ def run(args, mpi=True, **kw):
     return r.stdout
 
 
+def ubq():
+    """UBQ inputs and the hashes of the reference builder's .bat outputs."""
+    d = os.path.join(HERE, "ubq")
+    os.makedirs(d, exist_ok=True)
+    for ext in ("top", "xtc"):
+        shutil.copyfile(os.path.join(REFERENCE, "test_system", f"UBQ_UBM2.{ext}"), os.path.join(d, f"UBQ_UBM2.{ext}"))
+    lines = []
+    with tempfile.TemporaryDirectory() as td:
+        for tag, extra, bb in (("fp64", [], "CA C N H1 O1"), ("fp32", ["--single_precision"], "CA C N H1 O1"),
+                               ("fp64_bb_CA_N", [], "CA N")):
+            bat = os.path.join(td, f"{tag}.bat")
+            run([os.path.join(REF, "BAT_builder.x"), "-t", os.path.join(d, "UBQ_UBM2.top"), "-x",
+                 os.path.join(d, "UBQ_UBM2.xtc"), "-o", bat, "-bb", bb] + extra, mpi=False, cwd=td)
+            lines.append(f"{tag} {hashlib.sha256(open(bat, 'rb').read()).hexdigest()} {os.path.getsize(bat)}")
+    with open(os.path.join(d, "bat_sha256.txt"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
 def main():
+    if "--ubq-only" in sys.argv:
+        ubq()
+        return
     import parent_amd
 
     bat = os.path.join(HERE, "chain12_1000.bat")
@@ -56,6 +83,7 @@ def main():
     for name in ("MIE", "MIST"):
         shutil.copy(os.path.join(REFERENCE, "test_system", "sample_output", f"sample_output_{name}.txt"),
                     os.path.join(HERE, f"ubq_sample_{name}_tail.txt"))
+    ubq()
 
 
 if __name__ == "__main__":
