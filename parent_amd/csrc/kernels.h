@@ -68,7 +68,9 @@ struct AtomicConfig {
 };
 int atomic_config_count();
 AtomicConfig atomic_config(int id);
-int atomic_kind(int id);  // 0 plain LDS histograms, 1 sliced (bank-conflict-free), 2 packed sliced
+// 0 plain LDS histograms, 1 sliced (bank-conflict-free), 2 packed sliced,
+// 3 sliced with the perm + dot2 fill, 4 interleaved packed with that fill
+int atomic_kind(int id);
 size_t atomic_lds_bytes(int id);
 // Packed sliced histograms keep 16-bit counters of the frames of one lane
 // position (N / 16 + 16 of them at most).

@@ -77,6 +77,11 @@ __device__ __forceinline__ unsigned add_hi16(unsigned a, unsigned b) {
     return r;
 }
 
+// operand type of v_dot2_u32_u16 (__builtin_amdgcn_udot2)
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+// an LDS word addressed by its 32-bit LDS byte address
+typedef __attribute__((address_space(3))) unsigned lds_u32;
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -375,14 +380,27 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_atomic(PairInputs in0, TileQueu
 // The fill computes byte offsets straight from the index bytes (a 24-bit
 // multiply with a byte operand for the row, a shift for the column, one
 // three-way add), four frames per index word.
-template <int TR, int TC, int BLOCK, int SLOTC, int PACK, bool BATCH>
+//
+// DOT (1 x 1 tiles) forms each offset in two instructions: v_perm_b32 puts
+// the frame's row byte a and column byte b into the two 16-bit lanes of one
+// register, and v_dot2_u32_u16 returns a * rowmul + b * 64 + 4 s.  With
+// PACK = 2 it also interleaves the halves instead of pairing rows: cell
+// (a, b) is at byte a * S * 32 + b * 64 + 4 s (S = Bj rounded up to even),
+// in the low half for even a and the high half for odd a.  For each parity
+// the map (a, b, s) -> word is one-to-one, the word's bank is still
+// s + 16 * (word / 16 & 1), and the row byte needs no mask.  A slot holds
+// (Bi - 1) * S / 2 + Bj groups of 16 words; every half that no cell owns
+// stays 0.
+template <int TR, int TC, int BLOCK, int SLOTC, int PACK, bool DOT, bool BATCH>
 __global__ __launch_bounds__(BLOCK, 1) void k_pairs_sliced(PairInputs in0, TileQueue q,
                                                         int t_begin, int t_end,
                                                         const int* __restrict__ dump_begin,
                                                         const int* __restrict__ dump_entries,
                                                         int* __restrict__ dump_out) {
+    static_assert(!DOT || (TR == 1 && TC == 1), "DOT fill is for one-pair tiles");
     extern __shared__ unsigned hist[];
-    constexpr int kSlotW = SLOTC / PACK * 16;
+    // slot size: SLOTC cells, or SLOTC groups of 16 words for the interleaved halves
+    constexpr int kSlotW = (DOT && PACK == 2) ? SLOTC * 16 : SLOTC / PACK * 16;
     const int chunk = (t_end - t_begin + 7) / 8;
     const int t = t_begin + (int)(blockIdx.x % 8) * chunk + (int)(blockIdx.x / 8);
     if (t >= t_end || (int)(blockIdx.x / 8) >= chunk) return;
@@ -441,6 +459,7 @@ __global__ __launch_bounds__(BLOCK, 1) void k_pairs_sliced(PairInputs in0, TileQ
     // word of cell (a, b) in a slot with Bj columns, and the lane's increment
     auto cell_word = [&](unsigned a, unsigned b, unsigned Bj) {
         if constexpr (PACK == 1) return ((__umul24(a, Bj) + b) << 4) + s16;
+        else if constexpr (DOT) return (__umul24(a, (Bj + 1u) & ~1u) << 3) + (b << 4) + s16;
         else return ((__umul24(a >> 1, Bj) + b) << 4) + s16;
     };
     auto cell_inc = [&](unsigned a) {
@@ -459,7 +478,44 @@ __global__ __launch_bounds__(BLOCK, 1) void k_pairs_sliced(PairInputs in0, TileQ
         // shift amount and increment base for the byte-select shifts (VGPRs)
         unsigned six = 6u, one = 1u;
         asm("" : "+v"(six), "+v"(one));
+        // DOT: 16-bit lanes (row multiplier, 64) of the dot product, whose
+        // accumulator is the LDS byte address of the lane's sub-counter 0
+        const unsigned rowmul_dot = PACK == 1 ? (unsigned)Bc[0] * 64u : (((unsigned)Bc[0] + 1u) & ~1u) * 32u;
+        const unsigned kdot = rowmul_dot | (64u << 16);
+        const unsigned acc = (unsigned)(uintptr_t)(lds_u32*)hist + s4;
         auto process = [&](const uint4* R, const uint4* C) {
+            if constexpr (DOT) {
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const unsigned rw = comp(R[0], k), cw = comp(C[0], k);
+                    unsigned rsh = 0;
+                    if constexpr (PACK == 2) {
+                        rsh = (rw & 0x01010101u) << 4;
+                        asm("" : "+v"(rsh));
+                    }
+                    // the word's four addresses first: a dot product's result
+                    // is read by its atomic a few instructions later
+                    unsigned addr[4], inc[4];
+                    auto frame = [&](auto ec) {
+                        constexpr int e = decltype(ec)::value;
+                        // bytes: [row byte e, 0, column byte e, 0]
+                        constexpr unsigned sel = 0x0C000C00u | ((unsigned)e << 16) | (4u + (unsigned)e);
+                        const unsigned ab = __builtin_amdgcn_perm(rw, cw, sel);
+                        addr[e] = __builtin_amdgcn_udot2(__builtin_bit_cast(u16x2, ab),
+                                                         __builtin_bit_cast(u16x2, kdot), acc, false);
+                        inc[e] = PACK == 1 ? 1u : one_shl_byte<e>(rsh, one);
+                    };
+                    frame(std::integral_constant<int, 0>{});
+                    frame(std::integral_constant<int, 1>{});
+                    frame(std::integral_constant<int, 2>{});
+                    frame(std::integral_constant<int, 3>{});
+#pragma unroll
+                    for (int e = 0; e < 4; e++)
+                        __hip_atomic_fetch_add((lds_u32*)(uintptr_t)addr[e], inc[e], __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+                return;
+            }
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 unsigned rw[TR], rsh[TR], cw[TC];
@@ -522,18 +578,35 @@ __global__ __launch_bounds__(BLOCK, 1) void k_pairs_sliced(PairInputs in0, TileQ
             process(R0, C0);
             it++;
         }
-        it0 = it;
-        // the last, partial iteration: a thread whose 16 frames are all
-        // valid stays on the fast path; only the thread with the final
-        // partial chunk goes through the general loop below
-        if (it0 < niter && it0 * per_iter + (int)threadIdx.x * 16 + 16 <= N) {
-            load(R1, C1, it0);
-            process(R1, C1);
-            tail_fast = true;
+        // the last, partial iteration, one 4-frame index word per thread:
+        // the workgroup waits for its slowest thread, and 16-frame chunks
+        // would leave most threads idle here.  A lane position still counts
+        // at most N / 16 + 4 frames.
+        const int nw = (N + 3) / 4;
+        for (int w = nfull * (per_iter / 4) + (int)threadIdx.x; w < nw; w += BLOCK) {
+            unsigned rwv[TR], cwv[TC];
+#pragma unroll
+            for (int r = 0; r < TR; r++) rwv[r] = reinterpret_cast<const unsigned*>(rows[r])[w];
+#pragma unroll
+            for (int c = 0; c < TC; c++) cwv[c] = reinterpret_cast<const unsigned*>(cols[c])[w];
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                if (4 * w + e >= N) break;
+#pragma unroll
+                for (int r = 0; r < TR; r++) {
+                    const unsigned a = (rwv[r] >> (8 * e)) & 0xFFu;
+#pragma unroll
+                    for (int c = 0; c < TC; c++)
+                        atomicAdd(&hist[(r * TC + c) * kSlotW + cell_word(a, (cwv[c] >> (8 * e)) & 0xFFu,
+                                                                            (unsigned)Bc[c])],
+                                  cell_inc(a));
+                }
+            }
         }
+        tail_fast = true;
     }
     for (int it = it0; it < niter; it++) {
-        if (tail_fast) break;  // this thread's last iteration ran on the fast path
+        if (tail_fast) break;  // the fast path above covered every frame
         const int f = it * per_iter + threadIdx.x * 16;
         uint4 R[TR], C[TC];
 #pragma unroll
@@ -603,7 +676,8 @@ __global__ __launch_bounds__(BLOCK, 1) void k_pairs_sliced(PairInputs in0, TileQ
         // when packed), one per thread: thread t reads word (s + t) % 16 of
         // its group at step s, so the 16 lanes served together hit 16
         // distinct banks.
-        const int ngroups = (PACK == 1 ? Bi : (Bi + 1) / 2) * Bj;
+        const int ngroups = (DOT && PACK == 2) ? (Bi - 1) * (((Bj + 1) & ~1) / 2) + Bj
+                                               : (PACK == 1 ? Bi : (Bi + 1) / 2) * Bj;
         const uint4* hs4 = reinterpret_cast<const uint4*>(hist + sl * kSlotW);
         // a group is 4 uint4; thread t reads them in the order rotated by
         // (t / 2) % 4, so the 8 lanes a 128-byte read cycle serves (4 banks
@@ -611,6 +685,8 @@ __global__ __launch_bounds__(BLOCK, 1) void k_pairs_sliced(PairInputs in0, TileQ
         int quad[4];
 #pragma unroll
         for (int k = 0; k < 4; k++) quad[k] = (k + (threadIdx.x >> 1)) & 3;
+        // two groups per step: the table loads of both are in flight together
+#pragma unroll 2
         for (int g = threadIdx.x; g < ngroups; g += BLOCK) {
             unsigned lo = 0, hi = 0;
 #pragma unroll
@@ -628,8 +704,9 @@ __global__ __launch_bounds__(BLOCK, 1) void k_pairs_sliced(PairInputs in0, TileQ
             occ += lo != 0;
             if constexpr (PACK == 2) {
                 // the odd row of a packed pair exists unless Bi is odd and
-                // this is the last pair row (whose high halves stay 0)
-                const unsigned h2 = 2 * (g / Bj) + 1 < Bi ? hi : 0u;
+                // this is the last pair row (whose high halves stay 0);
+                // interleaved halves that no cell owns are 0 already
+                const unsigned h2 = DOT || 2 * (g / Bj) + 1 < Bi ? hi : 0u;
                 T += in.nlogn[h2];
                 occ += h2 != 0;
             }
@@ -662,7 +739,11 @@ namespace {
 // in cells (64 bytes each; 2500 -> B <= 50, 1225 -> B <= 35, 900 -> B <= 30,
 // 400 -> B <= 20, 100 -> B <= 10); kind 2: packed sliced histograms, slot
 // size in cells of B x (B rounded up to even) (32 bytes each; 2500 -> B <= 50,
-// 3600 -> B <= 60, 1296 -> B <= 36, 900 -> B <= 30).
+// 3600 -> B <= 60, 1296 -> B <= 36, 900 -> B <= 30); kind 3: sliced
+// histograms filled through v_perm + v_dot2, slot size in cells as kind 1;
+// kind 4: interleaved packed histograms filled the same way, slot size in
+// groups of 16 words, (B - 1) * S / 2 + B for B x B cells with S = B rounded
+// up to even (1275 -> B <= 50, 820 -> B <= 40, 465 -> B <= 30).
 #define PARENT_ATOMIC_VARIANTS(X)     \
     X(0, 0, 1, 1, 256, 2560)          \
     X(1, 0, 2, 2, 256, 2560)          \
@@ -703,7 +784,18 @@ namespace {
     X(36, 2, 1, 1, 512, 1296)         \
     X(37, 2, 1, 2, 512, 1296)         \
     X(38, 2, 2, 2, 1024, 900)         \
-    X(39, 2, 1, 1, 256, 900)
+    X(39, 2, 1, 1, 256, 900)          \
+    X(40, 3, 1, 1, 1024, 2500)        \
+    X(41, 3, 1, 1, 512, 2500)         \
+    X(42, 3, 1, 1, 1024, 900)         \
+    X(43, 3, 1, 1, 256, 900)          \
+    X(44, 3, 1, 1, 256, 400)          \
+    X(45, 4, 1, 1, 1024, 1275)        \
+    X(46, 4, 1, 1, 512, 1275)         \
+    X(47, 4, 1, 1, 256, 1275)         \
+    X(48, 4, 1, 1, 1024, 820)         \
+    X(49, 4, 1, 1, 256, 465)          \
+    X(50, 4, 1, 1, 1024, 465)
 
 struct AtomicVariant {
     AtomicConfig cfg;
@@ -716,22 +808,25 @@ struct AtomicVariant {
 template <int KIND, int TR, int TC, int BLK, int SL>
 hipError_t launch_variant(dim3 grid, size_t lds, hipStream_t s, const PairInputs& in, const TileQueue& q, int t0,
                           int t1, const int* db, const int* de, int* dout) {
-    constexpr int PACK = KIND == 2 ? 2 : 1;
+    constexpr int PACK = KIND == 2 || KIND == 4 ? 2 : 1;
+    constexpr bool DOT = KIND >= 3;
     const void* fn;
     if constexpr (KIND == 0)
         fn = q.descs ? reinterpret_cast<const void*>(&k_pairs_atomic<TR, TC, BLK, SL, true>)
                      : reinterpret_cast<const void*>(&k_pairs_atomic<TR, TC, BLK, SL, false>);
     else
-        fn = q.descs ? reinterpret_cast<const void*>(&k_pairs_sliced<TR, TC, BLK, SL, PACK, true>)
-                     : reinterpret_cast<const void*>(&k_pairs_sliced<TR, TC, BLK, SL, PACK, false>);
+        fn = q.descs ? reinterpret_cast<const void*>(&k_pairs_sliced<TR, TC, BLK, SL, PACK, DOT, true>)
+                     : reinterpret_cast<const void*>(&k_pairs_sliced<TR, TC, BLK, SL, PACK, DOT, false>);
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     if constexpr (KIND == 0) {
         if (q.descs) k_pairs_atomic<TR, TC, BLK, SL, true><<<grid, BLK, lds, s>>>(in, q, t0, t1, db, de, dout);
         else k_pairs_atomic<TR, TC, BLK, SL, false><<<grid, BLK, lds, s>>>(in, q, t0, t1, db, de, dout);
     } else {
-        if (q.descs) k_pairs_sliced<TR, TC, BLK, SL, PACK, true><<<grid, BLK, lds, s>>>(in, q, t0, t1, db, de, dout);
-        else k_pairs_sliced<TR, TC, BLK, SL, PACK, false><<<grid, BLK, lds, s>>>(in, q, t0, t1, db, de, dout);
+        if (q.descs)
+            k_pairs_sliced<TR, TC, BLK, SL, PACK, DOT, true><<<grid, BLK, lds, s>>>(in, q, t0, t1, db, de, dout);
+        else
+            k_pairs_sliced<TR, TC, BLK, SL, PACK, DOT, false><<<grid, BLK, lds, s>>>(in, q, t0, t1, db, de, dout);
     }
     return hipGetLastError();
 }
@@ -747,13 +842,18 @@ AtomicConfig atomic_config(int id) { return kAtomicVariants[id].cfg; }
 int atomic_kind(int id) { return kAtomicVariants[id].kind; }
 size_t atomic_lds_bytes(int id) {
     const AtomicVariant& v = kAtomicVariants[id];
-    const size_t per_slot = v.kind == 0 ? (size_t)v.slot * 4 : (v.kind == 1 ? (size_t)v.slot * 64 : (size_t)v.slot / 2 * 64);
+    const size_t per_slot = v.kind == 0 ? (size_t)v.slot * 4
+                            : v.kind == 2 ? (size_t)v.slot / 2 * 64
+                                          : (size_t)v.slot * 64;  // cells (1, 3) or 16-word groups (4)
     return (size_t)v.cfg.tr * v.cfg.tc * per_slot;
 }
 bool atomic_fits(int id, int bmax, size_t lds_max, int n_frames) {
     const AtomicVariant& v = kAtomicVariants[id];
-    const long long cells = v.kind == 2 ? (long long)bmax * ((bmax + 1) & ~1) : (long long)bmax * bmax;
-    if (v.kind == 2 && n_frames > kPackedMaxFrames) return false;
+    const long long even = (bmax + 1) & ~1;
+    const long long cells = v.kind == 2   ? (long long)bmax * even
+                            : v.kind == 4 ? (long long)(bmax - 1) * (even / 2) + bmax
+                                          : (long long)bmax * bmax;
+    if ((v.kind == 2 || v.kind == 4) && n_frames > kPackedMaxFrames) return false;
     return cells <= v.slot && atomic_lds_bytes(id) <= lds_max;
 }
 
