@@ -19,7 +19,7 @@ struct PrepareArgs {
     uint8_t* bins1;                   // [D][Np] or nullptr when 1D bins == 2D bins for all types
     double *s1d, *c2;                 // [D]
     double* jtab;                     // [D][256]
-    double* nlogn;                    // [N+1]
+    double* nlogn;                    // [N+1]: n ln n - 1/2, and 0 at n = 0
 };
 hipError_t launch_prepare(const PrepareArgs& a, hipStream_t s);
 // n ln n for n = 0..N (FP64 table of the fused epilogues)

@@ -9,8 +9,9 @@
 //                  d1 d2 + (occ-1)/(2N).  The baseline GPU kernel.
 //  k_pairs_atomic  a TR x TC tile of dof pairs per workgroup: the tile's row
 //                  and column index rows are streamed once and feed TR*TC LDS
-//                  histograms; fused epilogue S = ln N + c_i + c_j
-//                  - (1/N) sum n ln n + (occ-1)/(2N) over an FP64 n ln n table.
+//                  histograms; fused epilogue S = ln N + c_i + c_j - 1/(2N)
+//                  - (1/N) sum (n ln n - 1/2 [n > 0]) over an FP64 table of
+//                  that summand (the occupancy term (occ-1)/(2N) folded in).
 #include "kernels.h"
 
 #include <algorithm>
@@ -81,6 +82,17 @@ __device__ __forceinline__ unsigned add_hi16(unsigned a, unsigned b) {
 typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 // an LDS word addressed by its 32-bit LDS byte address
 typedef __attribute__((address_space(3))) unsigned lds_u32;
+
+// Raw buffer over one dof's index row (Np bytes): 16-byte loads at a
+// per-thread byte offset plus a uniform (SGPR) offset need no VALU address
+// math, and the hardware returns 0 past the row's end.  Word 3 is the raw
+// 32-bit data format of gfx9 buffer resources.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t index_row(const uint8_t* row, int Np) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<uint8_t*>(row), (short)0, Np, 0x00020000);
+}
+__device__ __forceinline__ uint4 load16(__amdgpu_buffer_rsrc_t rs, int voff, int soff) {
+    return __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rs, voff, soff, 0));
+}
 
 }  // namespace
 
@@ -339,20 +351,12 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_atomic(PairInputs in0, TileQueu
         const int cells = in.B.of(dof_type(nD, gi)) * Bc[c];
         const unsigned* h = hist + sl * SLOTW;
         double T = 0.0;
-        int occ = 0;
-        // branch-free (n ln n of 0 is 0) so the unrolled table loads are in
-        // flight together; same sums, in the same order, as skipping n = 0
+        // branch-free (the table is 0 at n = 0) so the unrolled table loads
+        // are in flight together
 #pragma unroll 4
-        for (int i = lane; i < cells; i += 64) {
-            const unsigned n = h[i];
-            T += in.nlogn[n];
-            occ += n != 0;
-        }
+        for (int i = lane; i < cells; i += 64) T += in.nlogn[h[i]];
         T = wave_total(T);
-        occ = wave_total(occ);
-        if (lane == 0)
-            in.s2d[par_offset(nD, gi, gj)] =
-                ((in.logN + in.c2[gi]) + in.c2[gj] + (occ - 1.0) / (2.0 * N)) - T / N;
+        if (lane == 0) in.s2d[par_offset(nD, gi, gj)] = ((in.logN + in.c2[gi]) + in.c2[gj] - 0.5 / N) - T / N;
     }
 }
 
@@ -441,13 +445,18 @@ __global__ __launch_bounds__(BLOCK, 1) void k_pairs_sliced(PairInputs in0, TileQ
     // The fast fill's first index words are requested before the histogram
     // clear, and the barrier after the clear waits for LDS operations only,
     // so the load latency of a new tile overlaps the clear.
+    __amdgpu_buffer_rsrc_t rrs[TR], crs[TC];
+#pragma unroll
+    for (int r = 0; r < TR; r++) rrs[r] = index_row(rows[r], in.Np);
+#pragma unroll
+    for (int c = 0; c < TC; c++) crs[c] = index_row(cols[c], in.Np);
+    const int toff = (int)threadIdx.x * 16;
     uint4 R0[TR], C0[TC];
     if (fast && nfull > 0) {
-        const int f = threadIdx.x * 16;
 #pragma unroll
-        for (int r = 0; r < TR; r++) R0[r] = *reinterpret_cast<const uint4*>(rows[r] + f);
+        for (int r = 0; r < TR; r++) R0[r] = load16(rrs[r], toff, 0);
 #pragma unroll
-        for (int c = 0; c < TC; c++) C0[c] = *reinterpret_cast<const uint4*>(cols[c] + f);
+        for (int c = 0; c < TC; c++) C0[c] = load16(crs[c], toff, 0);
     }
 
     {
@@ -557,13 +566,14 @@ __global__ __launch_bounds__(BLOCK, 1) void k_pairs_sliced(PairInputs in0, TileQ
         };
         // two register sets: the next iteration's index words load while
         // this iteration's atomics issue, without copies between the sets
+        // full iterations only: the partial last one goes by 4-frame words
         auto load = [&](uint4* R, uint4* C, int it) {
-            const int f = it * per_iter + threadIdx.x * 16;
-            if (f < in.Np) {
+            if (it < nfull) {
+                const int fb = it * per_iter;
 #pragma unroll
-                for (int r = 0; r < TR; r++) R[r] = *reinterpret_cast<const uint4*>(rows[r] + f);
+                for (int r = 0; r < TR; r++) R[r] = load16(rrs[r], toff, fb);
 #pragma unroll
-                for (int c = 0; c < TC; c++) C[c] = *reinterpret_cast<const uint4*>(cols[c] + f);
+                for (int c = 0; c < TC; c++) C[c] = load16(crs[c], toff, fb);
             }
         };
         uint4 R1[TR], C1[TC];  // R0 / C0 hold iteration 0 already
@@ -669,9 +679,7 @@ __global__ __launch_bounds__(BLOCK, 1) void k_pairs_sliced(PairInputs in0, TileQ
         const int Bi = in.B.of(dof_type(nD, gi)), Bj = Bc[c];
         // all waves share a slot: per-wave partial sums, then wave 0 combines
         __shared__ double sT[kWaves];
-        __shared__ int sOcc[kWaves];
         double T = 0.0;
-        int occ = 0;
         // Word groups (the 16 sub-counter words of a cell, or of a cell pair
         // when packed), one per thread: thread t reads word (s + t) % 16 of
         // its group at step s, so the 16 lanes served together hit 16
@@ -699,35 +707,27 @@ __global__ __launch_bounds__(BLOCK, 1) void k_pairs_sliced(PairInputs in0, TileQ
                     hi += add_hi16(v.x, v.y) + add_hi16(v.z, v.w);
                 }
             }
-            // branch-free: n ln n of 0 is 0, so both table loads issue together
+            // branch-free: the table is 0 at n = 0, so both loads issue together
             T += in.nlogn[lo];
-            occ += lo != 0;
             if constexpr (PACK == 2) {
                 // the odd row of a packed pair exists unless Bi is odd and
                 // this is the last pair row (whose high halves stay 0);
                 // interleaved halves that no cell owns are 0 already
                 const unsigned h2 = DOT || 2 * (g / Bj) + 1 < Bi ? hi : 0u;
                 T += in.nlogn[h2];
-                occ += h2 != 0;
             }
         }
         T = wave_total(T);
-        occ = wave_total(occ);
-        if (lane == 0) {
-            sT[wid] = T;
-            sOcc[wid] = occ;
-        }
+        if (lane == 0) sT[wid] = T;
         __syncthreads();
-        if (threadIdx.x == 0) {
-            double tt = 0.0;
-            int oo = 0;
-            for (int w = 0; w < kWaves; w++) {
-                tt += sT[w];
-                oo += sOcc[w];
-            }
-            in.s2d[par_offset(nD, gi, gj)] = ((in.logN + in.c2[gi]) + in.c2[gj] + (oo - 1.0) / (2.0 * N)) - tt / N;
+        if (wid == 0) {
+            // the waves' partial sums in one cross-lane reduction (fixed
+            // order): the workgroup, and its LDS, ends sooner than with a
+            // serial loop
+            const double tt = wave_total(lane < kWaves ? sT[lane] : 0.0);
+            if (lane == 0) in.s2d[par_offset(nD, gi, gj)] = ((in.logN + in.c2[gi]) + in.c2[gj] - 0.5 / N) - tt / N;
         }
-        __syncthreads();
+        if (sl + 1 < TR * TC) __syncthreads();  // sT is reused by the next slot
     }
 }
 

@@ -192,17 +192,11 @@ __global__ __launch_bounds__(256, 2) void k_pairs_mfma(PairInputs in0, int B, in
         const int gi = r0 + di, gj = c0 + dj;
         if (!(gi < gj && gj < D)) continue;
         double T = 0.0;
-        int occ = 0;
-        // branch-free (n ln n of 0 is 0): the unrolled table loads overlap
+        // branch-free (the table is 0 at n = 0): the unrolled table loads overlap
 #pragma unroll 4
-        for (int c = lane; c < B * B; c += 64) {
-            const int n = cnt[(di * B + c / B) * kTile + dj * B + c % B];
-            T += in.nlogn[n];
-            occ += n != 0;
-        }
+        for (int c = lane; c < B * B; c += 64) T += in.nlogn[cnt[(di * B + c / B) * kTile + dj * B + c % B]];
         T = wave_total(T);
-        occ = wave_total(occ);
-        if (lane == 0) in.s2d[par_offset(nD, gi, gj)] = ((in.logN + in.c2[gi]) + in.c2[gj] + (occ - 1.0) / (2.0 * N)) - T / N;
+        if (lane == 0) in.s2d[par_offset(nD, gi, gj)] = ((in.logN + in.c2[gi]) + in.c2[gj] - 0.5 / N) - T / N;
     }
 }
 

@@ -401,10 +401,14 @@ __global__ __launch_bounds__(256) void k_dof_stats(const uint8_t* __restrict__ b
     }
 }
 
-// n*ln(n) for n = 0..N in FP64: the 2D epilogue sums table entries.
+// n ln n - 1/2 for n = 1..N and 0 for n = 0, in FP64.  The 2D epilogue sums
+// table entries over the cells: the occupancy term (occ - 1) / (2N) of
+// S_ij then comes out of the same sum, S_ij = ln N + c_i + c_j - 1/(2N)
+// - (1/N) sum (n ln n - 1/2 [n > 0]), with no separate count of occupied
+// cells.
 __global__ void k_nlogn(int N, double* tab) {
     for (int n = blockIdx.x * blockDim.x + threadIdx.x; n <= N; n += gridDim.x * blockDim.x)
-        tab[n] = n == 0 ? 0.0 : (double)n * log((double)n);
+        tab[n] = n == 0 ? 0.0 : (double)n * log((double)n) - 0.5;
 }
 
 // ---------------------------------------------------------------------------
