@@ -464,6 +464,15 @@ __global__ __launch_bounds__(BLOCK, 1) void k_pairs_sliced(PairInputs in0, TileQ
         for (int i = threadIdx.x; i < TR * TC * kSlotW / 4; i += BLOCK) h4[i] = make_uint4(0, 0, 0, 0);
     }
     lds_barrier();
+    // c2 of the tile's dofs for the epilogue: scalar loads (the constant
+    // address space; c2 is not written while pairs run), issued after the
+    // barrier so that it does not wait for them, and in flight during the fill
+    typedef __attribute__((address_space(4))) const double cdouble;
+    double c2r[TR], c2c[TC];
+#pragma unroll
+    for (int r = 0; r < TR; r++) c2r[r] = ((cdouble*)in.c2)[gr[r] < D ? gr[r] : 0];
+#pragma unroll
+    for (int c = 0; c < TC; c++) c2c[c] = ((cdouble*)in.c2)[gc[c] < D ? gc[c] : 0];
 
     // word of cell (a, b) in a slot with Bj columns, and the lane's increment
     auto cell_word = [&](unsigned a, unsigned b, unsigned Bj) {
@@ -693,30 +702,38 @@ __global__ __launch_bounds__(BLOCK, 1) void k_pairs_sliced(PairInputs in0, TileQ
         int quad[4];
 #pragma unroll
         for (int k = 0; k < 4; k++) quad[k] = (k + (threadIdx.x >> 1)) & 3;
-        // two groups per step: the table loads of both are in flight together
-#pragma unroll 2
-        for (int g = threadIdx.x; g < ngroups; g += BLOCK) {
-            unsigned lo = 0, hi = 0;
+        // The thread's groups (at most kMaxG) are counted first and their
+        // table loads issued together, then summed in group order, low half
+        // then high half; a group past the end counts 0, whose entry is 0.
+        constexpr int kMaxG = (kSlotW / 16 + BLOCK - 1) / BLOCK;
+        unsigned cnt[2 * kMaxG];
 #pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const uint4 v = hs4[g * 4 + quad[k]];
-                if constexpr (PACK == 1) {
-                    lo += (v.x + v.y) + (v.z + v.w);
-                } else {
-                    lo += add_lo16(v.x, v.y) + add_lo16(v.z, v.w);
-                    hi += add_hi16(v.x, v.y) + add_hi16(v.z, v.w);
+        for (int k = 0; k < kMaxG; k++) {
+            const int g = (int)threadIdx.x + k * BLOCK;
+            unsigned lo = 0, hi = 0;
+            if (g < ngroups) {
+#pragma unroll
+                for (int kk = 0; kk < 4; kk++) {
+                    const uint4 v = hs4[g * 4 + quad[kk]];
+                    if constexpr (PACK == 1) {
+                        lo += (v.x + v.y) + (v.z + v.w);
+                    } else {
+                        lo += add_lo16(v.x, v.y) + add_lo16(v.z, v.w);
+                        hi += add_hi16(v.x, v.y) + add_hi16(v.z, v.w);
+                    }
                 }
             }
-            // branch-free: the table is 0 at n = 0, so both loads issue together
-            T += in.nlogn[lo];
-            if constexpr (PACK == 2) {
-                // the odd row of a packed pair exists unless Bi is odd and
-                // this is the last pair row (whose high halves stay 0);
-                // interleaved halves that no cell owns are 0 already
-                const unsigned h2 = DOT || 2 * (g / Bj) + 1 < Bi ? hi : 0u;
-                T += in.nlogn[h2];
-            }
+            cnt[2 * k] = lo;
+            // the odd row of a packed pair exists unless Bi is odd and this
+            // is the last pair row (whose high halves stay 0); interleaved
+            // halves that no cell owns are 0 already
+            cnt[2 * k + 1] = PACK == 1 ? 0u : (DOT || 2 * (g / Bj) + 1 < Bi ? hi : 0u);
         }
+        double tab[2 * kMaxG];
+#pragma unroll
+        for (int k = 0; k < 2 * kMaxG; k++) tab[k] = (PACK == 1 && (k & 1)) ? 0.0 : in.nlogn[cnt[k]];
+#pragma unroll
+        for (int k = 0; k < 2 * kMaxG; k++) T += tab[k];
         T = wave_total(T);
         if (lane == 0) sT[wid] = T;
         __syncthreads();
@@ -725,7 +742,7 @@ __global__ __launch_bounds__(BLOCK, 1) void k_pairs_sliced(PairInputs in0, TileQ
             // order): the workgroup, and its LDS, ends sooner than with a
             // serial loop
             const double tt = wave_total(lane < kWaves ? sT[lane] : 0.0);
-            if (lane == 0) in.s2d[par_offset(nD, gi, gj)] = ((in.logN + in.c2[gi]) + in.c2[gj] - 0.5 / N) - tt / N;
+            if (lane == 0) in.s2d[par_offset(nD, gi, gj)] = ((in.logN + c2r[r]) + c2c[c] - 0.5 / N) - tt / N;
         }
         if (sl + 1 < TR * TC) __syncthreads();  // sT is reused by the next slot
     }
@@ -850,10 +867,11 @@ size_t atomic_lds_bytes(int id) {
 bool atomic_fits(int id, int bmax, size_t lds_max, int n_frames) {
     const AtomicVariant& v = kAtomicVariants[id];
     const long long even = (bmax + 1) & ~1;
+    const bool halves = v.kind == 2 || v.kind == 4;
     const long long cells = v.kind == 2   ? (long long)bmax * even
                             : v.kind == 4 ? (long long)(bmax - 1) * (even / 2) + bmax
                                           : (long long)bmax * bmax;
-    if ((v.kind == 2 || v.kind == 4) && n_frames > kPackedMaxFrames) return false;
+    if (halves && n_frames > kPackedMaxFrames) return false;
     return cells <= v.slot && atomic_lds_bytes(id) <= lds_max;
 }
 
