@@ -139,7 +139,11 @@ def test_every_atomic_variant_bit_exact(engine):
     for bins, atoms, frames in (([50] * 6, 26, 4097), ([10, 10, 10, 20, 30, 40], 22, 3000),
                                 ([75] * 6, 14, 2500), ([150] * 6, 10, 1500), ([10] * 6, 20, 2000),
                                 ([30] * 6, 18, 2000), ([20] * 6, 16, 17000),
-                                ([7, 13, 29, 5, 40, 17], 18, 3000), ([35] * 6, 12, 2500), ([59] * 6, 10, 2000)):
+                                ([7, 13, 29, 5, 40, 17], 18, 3000), ([35] * 6, 12, 2500), ([59] * 6, 10, 2000),
+                                # frames past one iteration of the 1,024-thread fills (16,384
+                                # frames): the perm + dot2 loop, its prefetch and the 4-frame tail
+                                ([50] * 6, 10, 40000), ([35] * 6, 10, 33001),
+                                ([7, 13, 29, 5, 40, 17], 10, 20002)):
         vals, nD = _case(atoms, frames, 13)
         engine.set_trajectory(vals)
         engine.prepare(bins)
@@ -222,12 +226,15 @@ def test_mfma_out_of_range_falls_back(engine):
     assert engine.stats()["method"] == pa.METHOD_ATOMIC
 
 
-def test_packed_counters_at_the_frame_limit(engine):
+@pytest.mark.parametrize("variant,bins", [(31, 50), (45, 50), (49, 30)])
+def test_packed_counters_at_the_frame_limit(engine, variant, bins):
     """The packed sliced histograms keep 16-bit counters of the frames of one
     lane position.  A constant trajectory puts every frame of every pair in
     one cell, so at the largest frame count the packed variants accept
     (1,048,304) one counter reaches 65,519: the counts must still be exact.
-    One frame more and the engine must not pick a packed variant."""
+    One frame more and the engine must not pick a packed variant.  Variant 31
+    pairs rows in a word; 45 (50 bins) and 49 (30 bins) interleave the halves
+    by row parity."""
     pa = _pa()
     n_max = (65535 - 16) * 16
     nD = 3
@@ -236,22 +243,22 @@ def test_packed_counters_at_the_frame_limit(engine):
     for frames, packed_ok in ((n_max, True), (n_max + 1, False)):
         vals = np.tile(row, (frames, 1))
         engine.set_trajectory(vals)
-        engine.prepare([50] * 6)
+        engine.prepare([bins] * 6)
         engine.pairs(pa.METHOD_PLAIN)
         s2p = engine.s2d()
-        engine.set_variant(pa.METHOD_ATOMIC, 31)  # packed sliced, 1 x 1 tiles
+        engine.set_variant(pa.METHOD_ATOMIC, variant)
         engine.pairs(pa.METHOD_ATOMIC)
         st = engine.stats()
         engine.set_variant(pa.METHOD_ATOMIC, -1)
-        print(f"packed limit: frames {frames} method {st['method']} variant {st['kernel_variant']}", flush=True)
+        print(f"packed limit: variant {variant} frames {frames} method {st['method']} variant {st['kernel_variant']}", flush=True)
         if not packed_ok:
             assert st["method"] == pa.METHOD_PLAIN  # the forced packed variant does not fit
             continue
-        assert st["method"] == pa.METHOD_ATOMIC and st["kernel_variant"] == 31
+        assert st["method"] == pa.METHOD_ATOMIC and st["kernel_variant"] == variant
         assert np.array_equal(engine.s2d().view(np.uint64), s2p.view(np.uint64)) or \
             np.max(np.abs(engine.s2d() - s2p)) <= S_ABS
-        engine.set_variant(pa.METHOD_ATOMIC, 31)
-        h = engine.pair_counts([(0, 1), (0, D - 1), (D - 2, D - 1)], pa.METHOD_ATOMIC, [50, 50, 50])
+        engine.set_variant(pa.METHOD_ATOMIC, variant)
+        h = engine.pair_counts([(0, 1), (0, D - 1), (D - 2, D - 1)], pa.METHOD_ATOMIC, [bins] * 3)
         engine.set_variant(pa.METHOD_ATOMIC, -1)
         for x in h:
             assert x.sum() == frames and x.max() == frames
