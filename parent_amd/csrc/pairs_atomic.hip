@@ -748,6 +748,247 @@ __global__ __launch_bounds__(BLOCK, 1) void k_pairs_sliced(PairInputs in0, TileQ
     }
 }
 
+// ---------------------------------------------------------------------------
+// multi-tile sliced kernel: one-pair tiles, perm + dot2 fill, TPW tiles per workgroup
+// ---------------------------------------------------------------------------
+// A workgroup takes TPW consecutive tiles of its XCD's range, with the
+// interleaved (PACK = 2) or 32-bit (PACK = 1) layout of the DOT sliced
+// kernel.  Per tile: fill; barrier; an epilogue that zeroes each group it
+// reads (the next tile needs no clear); the next tile's queue entry and first
+// index words requested while this tile's table loads are in flight;
+// barrier; the tile's cross-wave sum on wave 0 while the other waves start
+// the next fill.  Workgroups still start when slots free, at the
+// dispatcher's times, so the two workgroups of a CU do not fall into step.
+// Register use decides the speed here: a version that spilled to scratch
+// (the tile loop unrolled, the kernel-argument inputs copied per tile) ran
+// at half the rate, so the loop stays rolled and one trajectory's inputs are
+// read from the argument in place.
+template <int BLOCK, int SLOTC, int PACK, int TPW, bool BATCH>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(BLOCK / 128))) void k_pairs_multi(
+    PairInputs in0, TileQueue q, int t_begin, int t_end, const int* __restrict__ dump_begin,
+    const int* __restrict__ dump_entries, int* __restrict__ dump_out) {
+    extern __shared__ unsigned hist[];
+    constexpr int kWaves = BLOCK / 64;
+    constexpr int kSlotW = SLOTC * 16;                   // 16-word groups (PACK 2) or cells (PACK 1)
+    constexpr int kMaxG = (SLOTC + BLOCK - 1) / BLOCK;   // groups per thread
+    constexpr int per_iter = BLOCK * 16;
+    __shared__ double sT[2][kWaves];
+    const int chunk = (t_end - t_begin + 7) / 8;        // tiles of this XCD
+    const int tbase = t_begin + (int)(blockIdx.x % 8) * chunk;
+    const int l0 = (int)(blockIdx.x / 8) * TPW;
+    auto has = [&](int local) { return local < chunk && tbase + local < t_end; };
+    if (!has(l0)) return;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const unsigned s16 = threadIdx.x & 15;
+    const int toff = (int)threadIdx.x * 16;
+    const unsigned acc = (unsigned)(uintptr_t)(lds_u32*)hist + s16 * 4u;
+    uint4* h4 = reinterpret_cast<uint4*>(hist);
+    unsigned one = 1u;
+    asm("" : "+v"(one));
+    int quad[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) quad[k] = (k + (threadIdx.x >> 1)) & 3;
+
+    // the queue and the batch descriptors are not written while the kernel
+    // runs: read through the constant address space they are scalar loads
+    typedef __attribute__((address_space(4))) const int qint;
+    typedef __attribute__((address_space(4))) const unsigned qword;
+    struct TileIdx {
+        int t, gi, gj, k;
+    };
+    auto fetch = [&](int local) {
+        TileIdx x;
+        x.t = tbase + local;
+        qint* tp = (qint*)(q.tiles + (size_t)(BATCH ? 3 : 2) * x.t);
+        x.gi = tp[0];
+        x.gj = tp[1];
+        x.k = BATCH ? tp[2] : 0;
+        return x;
+    };
+    auto inputs = [&](const TileIdx& x) -> PairInputs {
+        if constexpr (BATCH) {
+            static_assert(sizeof(PairInputs) % 4 == 0, "descriptor copied as 32-bit words");
+            qword* src = (qword*)(q.descs + x.k);
+            PairInputs r;
+            unsigned* dst = reinterpret_cast<unsigned*>(&r);
+#pragma unroll
+            for (int i = 0; i < (int)(sizeof(PairInputs) / 4); i++) dst[i] = src[i];
+            return r;
+        } else {
+            return in0;
+        }
+    };
+    auto cell_word = [&](unsigned a, unsigned b, unsigned Bj) {
+        if constexpr (PACK == 1) return ((__umul24(a, Bj) + b) << 4) + s16;
+        else return (__umul24(a, (Bj + 1u) & ~1u) << 3) + (b << 4) + s16;
+    };
+    auto cell_inc = [&](unsigned a) {
+        if constexpr (PACK == 1) return 1u;
+        else return 1u << ((a & 1u) << 4);
+    };
+    auto first_words = [&](const PairInputs& in, const TileIdx& x, uint4& R, uint4& C) {
+        if (x.gi < x.gj && x.gj < in.D && in.N >= per_iter) {
+            R = load16(index_row(in.bins + (size_t)x.gi * in.Np, in.Np), toff, 0);
+            C = load16(index_row(in.bins + (size_t)x.gj * in.Np, in.Np), toff, 0);
+        }
+    };
+
+    int local = l0;
+    TileIdx x = fetch(local);
+    PairInputs in = inputs(x);
+    uint4 R0, C0;
+    first_words(in, x, R0, C0);
+    for (int i = threadIdx.x; i < kSlotW / 4; i += BLOCK) h4[i] = make_uint4(0, 0, 0, 0);
+    lds_barrier();
+    int par = 0;
+    // not unrolled: one tile's state at a time stays in registers
+#pragma unroll 1
+    for (int k = 0; k < TPW; k++) {
+        const int nlocal = local + 1;
+        const bool more = k + 1 < TPW && has(nlocal);
+        const TileIdx nx = more ? fetch(nlocal) : x;
+        const bool valid = x.gi < x.gj && x.gj < in.D;
+        const int Bi = valid ? in.B.of(dof_type(in.nD, x.gi)) : 1;
+        const int Bj = valid ? in.B.of(dof_type(in.nD, x.gj)) : 1;
+        // ---- fill
+        if (valid) {
+            const int N = in.N, nfull = N / per_iter;
+            const uint8_t* rowp = in.bins + (size_t)x.gi * in.Np;
+            const uint8_t* colp = in.bins + (size_t)x.gj * in.Np;
+            const __amdgpu_buffer_rsrc_t rr = index_row(rowp, in.Np), cr = index_row(colp, in.Np);
+            const unsigned rowmul = PACK == 1 ? (unsigned)Bj * 64u : (((unsigned)Bj + 1u) & ~1u) * 32u;
+            const unsigned kdot = rowmul | (64u << 16);
+            auto process = [&](const uint4& R, const uint4& C) {
+#pragma unroll
+                for (int kk = 0; kk < 4; kk++) {
+                    const unsigned rw = kk == 0 ? R.x : (kk == 1 ? R.y : (kk == 2 ? R.z : R.w));
+                    const unsigned cw = kk == 0 ? C.x : (kk == 1 ? C.y : (kk == 2 ? C.z : C.w));
+                    unsigned rsh = 0;
+                    if constexpr (PACK == 2) {
+                        rsh = (rw & 0x01010101u) << 4;
+                        asm("" : "+v"(rsh));
+                    }
+                    unsigned addr[4], inc[4];
+                    auto frame = [&](auto ec) {
+                        constexpr int e = decltype(ec)::value;
+                        constexpr unsigned sel = 0x0C000C00u | ((unsigned)e << 16) | (4u + (unsigned)e);
+                        const unsigned ab = __builtin_amdgcn_perm(rw, cw, sel);
+                        addr[e] = __builtin_amdgcn_udot2(__builtin_bit_cast(u16x2, ab),
+                                                         __builtin_bit_cast(u16x2, kdot), acc, false);
+                        inc[e] = PACK == 1 ? 1u : one_shl_byte<e>(rsh, one);
+                    };
+                    frame(std::integral_constant<int, 0>{});
+                    frame(std::integral_constant<int, 1>{});
+                    frame(std::integral_constant<int, 2>{});
+                    frame(std::integral_constant<int, 3>{});
+#pragma unroll
+                    for (int e = 0; e < 4; e++)
+                        __hip_atomic_fetch_add((lds_u32*)(uintptr_t)addr[e], inc[e], __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+            };
+            uint4 R1, C1;
+            int it = 0;
+            for (; it + 2 <= nfull; it += 2) {
+                R1 = load16(rr, toff, (it + 1) * per_iter);
+                C1 = load16(cr, toff, (it + 1) * per_iter);
+                process(R0, C0);
+                if (it + 2 < nfull) {
+                    R0 = load16(rr, toff, (it + 2) * per_iter);
+                    C0 = load16(cr, toff, (it + 2) * per_iter);
+                }
+                process(R1, C1);
+            }
+            if (it < nfull) process(R0, C0);
+            // the partial last iteration, one 4-frame index word per thread
+            const int nw = (N + 3) / 4;
+            for (int w = nfull * (per_iter / 4) + (int)threadIdx.x; w < nw; w += BLOCK) {
+                const unsigned rw = reinterpret_cast<const unsigned*>(rowp)[w];
+                const unsigned cw = reinterpret_cast<const unsigned*>(colp)[w];
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    if (4 * w + e >= N) break;
+                    const unsigned a = (rw >> (8 * e)) & 0xFFu, b = (cw >> (8 * e)) & 0xFFu;
+                    atomicAdd(&hist[cell_word(a, b, (unsigned)Bj)], cell_inc(a));
+                }
+            }
+        }
+        lds_barrier();
+        // ---- counts of the tile's cells (parity checks), before the zeroing epilogue
+        if (dump_out && valid) {
+            for (int e = dump_begin[x.t]; e < dump_begin[x.t + 1]; e++) {
+                const long long off = ((long long)dump_entries[3 * e + 1] << 31) | (long long)dump_entries[3 * e + 2];
+                for (int i = threadIdx.x; i < Bi * Bj; i += BLOCK) {
+                    const unsigned a = (unsigned)(i / Bj), b = (unsigned)(i - (int)a * Bj);
+                    const uint4* g4 = reinterpret_cast<const uint4*>(hist + (cell_word(a, b, (unsigned)Bj) - s16));
+                    const unsigned sh = PACK == 2 ? (a & 1u) * 16u : 0u;
+                    const unsigned m = PACK == 2 ? 0xFFFFu : 0xFFFFFFFFu;
+                    unsigned n = 0;
+#pragma unroll
+                    for (int kk = 0; kk < 4; kk++) {
+                        const uint4 v = g4[kk];
+                        n += ((v.x >> sh) & m) + ((v.y >> sh) & m) + ((v.z >> sh) & m) + ((v.w >> sh) & m);
+                    }
+                    dump_out[off + i] = (int)n;
+                }
+            }
+            __syncthreads();
+        }
+        // ---- epilogue: read and zero this thread's groups
+        const int ngroups = !valid ? 0 : PACK == 2 ? (Bi - 1) * (((Bj + 1) & ~1) / 2) + Bj : Bi * Bj;
+        unsigned cnt[2 * kMaxG];
+#pragma unroll
+        for (int kk = 0; kk < kMaxG; kk++) {
+            const int g = (int)threadIdx.x + kk * BLOCK;
+            unsigned lo = 0, hi = 0;
+            if (g < ngroups) {
+#pragma unroll
+                for (int q4 = 0; q4 < 4; q4++) {
+                    const uint4 v = h4[g * 4 + quad[q4]];
+                    if (more) h4[g * 4 + quad[q4]] = make_uint4(0, 0, 0, 0);
+                    if constexpr (PACK == 1) {
+                        lo += (v.x + v.y) + (v.z + v.w);
+                    } else {
+                        lo += add_lo16(v.x, v.y) + add_lo16(v.z, v.w);
+                        hi += add_hi16(v.x, v.y) + add_hi16(v.z, v.w);
+                    }
+                }
+            }
+            cnt[2 * kk] = lo;
+            cnt[2 * kk + 1] = hi;
+        }
+        // this tile's table loads, then the next tile's first index words
+        double tab[2 * kMaxG];
+#pragma unroll
+        for (int kk = 0; kk < 2 * kMaxG; kk++) tab[kk] = (PACK == 1 && (kk & 1)) ? 0.0 : in.nlogn[cnt[kk]];
+        // (a batch tile's inputs are its trajectory's descriptor; one
+        // trajectory's are the kernel argument, never copied)
+        PairInputs nin = in;
+        if constexpr (BATCH) {
+            if (more) nin = inputs(nx);
+        }
+        if (more) first_words(nin, nx, R0, C0);
+        // groups in order, low half then high half (unused halves add 0)
+        double T = 0.0;
+#pragma unroll
+        for (int kk = 0; kk < 2 * kMaxG; kk++) T += tab[kk];
+        T = wave_total(T);
+        if (lane == 0) sT[par][wid] = T;
+        lds_barrier();
+        if (wid == 0 && valid) {
+            const double tt = wave_total(lane < kWaves ? sT[par][lane] : 0.0);
+            if (lane == 0)
+                in.s2d[par_offset(in.nD, x.gi, x.gj)] =
+                    ((in.logN + in.c2[x.gi]) + in.c2[x.gj] - 0.5 / in.N) - tt / in.N;
+        }
+        if (!more) break;
+        x = nx;
+        if constexpr (BATCH) in = nin;
+        local = nlocal;
+        par ^= 1;
+    }
+}
+
 namespace {
 
 // (kind, tile rows, tile columns, threads, slot size).  kind 0: plain LDS
@@ -760,7 +1001,9 @@ namespace {
 // histograms filled through v_perm + v_dot2, slot size in cells as kind 1;
 // kind 4: interleaved packed histograms filled the same way, slot size in
 // groups of 16 words, (B - 1) * S / 2 + B for B x B cells with S = B rounded
-// up to even (1275 -> B <= 50, 820 -> B <= 40, 465 -> B <= 30).
+// up to even (1275 -> B <= 50, 820 -> B <= 40, 465 -> B <= 30); kinds 5
+// and 6: the layout of kind 4 in k_pairs_multi, 2 consecutive tiles per
+// workgroup (5) or 1 (6).
 #define PARENT_ATOMIC_VARIANTS(X)     \
     X(0, 0, 1, 1, 256, 2560)          \
     X(1, 0, 2, 2, 256, 2560)          \
@@ -812,7 +1055,12 @@ namespace {
     X(47, 4, 1, 1, 256, 1275)         \
     X(48, 4, 1, 1, 1024, 820)         \
     X(49, 4, 1, 1, 256, 465)          \
-    X(50, 4, 1, 1, 1024, 465)
+    X(50, 4, 1, 1, 1024, 465)         \
+    X(51, 5, 1, 1, 256, 465)          \
+    X(52, 5, 1, 1, 256, 1275)         \
+    X(53, 6, 1, 1, 1024, 1275)        \
+    X(54, 6, 1, 1, 1024, 820)         \
+    X(55, 6, 1, 1, 256, 465)
 
 struct AtomicVariant {
     AtomicConfig cfg;
@@ -825,6 +1073,19 @@ struct AtomicVariant {
 template <int KIND, int TR, int TC, int BLK, int SL>
 hipError_t launch_variant(dim3 grid, size_t lds, hipStream_t s, const PairInputs& in, const TileQueue& q, int t0,
                           int t1, const int* db, const int* de, int* dout) {
+    if constexpr (KIND >= 5) {
+        constexpr int TPW = KIND == 5 ? 2 : 1;
+        const void* mf = q.descs ? reinterpret_cast<const void*>(&k_pairs_multi<BLK, SL, 2, TPW, true>)
+                                 : reinterpret_cast<const void*>(&k_pairs_multi<BLK, SL, 2, TPW, false>);
+        hipError_t e = hipFuncSetAttribute(mf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+        // TPW consecutive tiles of an XCD's range per workgroup
+        const int chunk = (t1 - t0 + 7) / 8;
+        const dim3 g((unsigned)((chunk + TPW - 1) / TPW) * 8);
+        if (q.descs) k_pairs_multi<BLK, SL, 2, TPW, true><<<g, BLK, lds, s>>>(in, q, t0, t1, db, de, dout);
+        else k_pairs_multi<BLK, SL, 2, TPW, false><<<g, BLK, lds, s>>>(in, q, t0, t1, db, de, dout);
+        return hipGetLastError();
+    }
     constexpr int PACK = KIND == 2 || KIND == 4 ? 2 : 1;
     constexpr bool DOT = KIND >= 3;
     const void* fn;
@@ -861,21 +1122,24 @@ size_t atomic_lds_bytes(int id) {
     const AtomicVariant& v = kAtomicVariants[id];
     const size_t per_slot = v.kind == 0 ? (size_t)v.slot * 4
                             : v.kind == 2 ? (size_t)v.slot / 2 * 64
-                                          : (size_t)v.slot * 64;  // cells (1, 3) or 16-word groups (4)
+                                          : (size_t)v.slot * 64;  // cells (1, 3) or 16-word groups (4, 5)
     return (size_t)v.cfg.tr * v.cfg.tc * per_slot;
 }
 bool atomic_fits(int id, int bmax, size_t lds_max, int n_frames) {
     const AtomicVariant& v = kAtomicVariants[id];
     const long long even = (bmax + 1) & ~1;
-    const bool halves = v.kind == 2 || v.kind == 4;
-    const long long cells = v.kind == 2   ? (long long)bmax * even
-                            : v.kind == 4 ? (long long)(bmax - 1) * (even / 2) + bmax
-                                          : (long long)bmax * bmax;
+    const bool halves = v.kind == 2 || v.kind >= 4;
+    const long long cells = v.kind == 2  ? (long long)bmax * even
+                            : v.kind >= 4 ? (long long)(bmax - 1) * (even / 2) + bmax
+                                                         : (long long)bmax * bmax;
     if (halves && n_frames > kPackedMaxFrames) return false;
     return cells <= v.slot && atomic_lds_bytes(id) <= lds_max;
 }
 
-int atomic_tiles_per_launch(int id) { return tiles_per_launch(kAtomicVariants[id].cfg.block); }
+// kind 5 covers 2 tiles per workgroup of a launch
+int atomic_tiles_per_launch(int id) {
+    return tiles_per_launch(kAtomicVariants[id].cfg.block) * (kAtomicVariants[id].kind == 5 ? 2 : 1);
+}
 
 hipError_t launch_pairs_atomic(const PairInputs& in, int id, const TileQueue& q,
                                const int* dump_begin, const int* dump_entries, int* dump_out,
