@@ -55,9 +55,9 @@ HBM_PEAK = 8.0e12
 LDS_PEAK_FALLBACK = {"conflict_free": 9.5e12, "random_2500": 5.2e12}  # lds_peak.x on MI355X
 # HBM bytes per pass of the pair kernel, from rocprofv3 FETCH_SIZE and WRITE_SIZE passes scaled
 # by the elementwise-kernel calibration (FETCH x 2.00, WRITE x 1.00):
-# profiles/pairs_sliced_interleaved_protein_1500x100k_b50.md (k_pairs_sliced<1, 1, 1024, 1275, 2,
-# true>, the autotuner's choice at 50 bins: 63.63 GB read + 0.164 GB written).
-PROFILED_TRAFFIC = {("protein_1500x100k_b50", 1): 63.63e9 + 0.164e9}
+# profiles/pairs_multi_protein_1500x100k_b50.md (k_pairs_multi<1024, 1275, 2, 1>, the autotuner's
+# choice at 50 bins: 63.62 GB read + 0.166 GB written).
+PROFILED_TRAFFIC = {("protein_1500x100k_b50", 1): 63.62e9 + 0.166e9}
 METRIC = "pair-frame updates/s, all-pairs MI (4,491 dofs,100k fr,50 bins): tuned vs plain"
 
 
