@@ -226,7 +226,7 @@ def test_mfma_out_of_range_falls_back(engine):
     assert engine.stats()["method"] == pa.METHOD_ATOMIC
 
 
-@pytest.mark.parametrize("variant,bins", [(31, 50), (45, 50), (49, 30)])
+@pytest.mark.parametrize("variant,bins", [(31, 50), (45, 50), (49, 30), (53, 50), (51, 30)])
 def test_packed_counters_at_the_frame_limit(engine, variant, bins):
     """The packed sliced histograms keep 16-bit counters of the frames of one
     lane position.  A constant trajectory puts every frame of every pair in
@@ -234,7 +234,7 @@ def test_packed_counters_at_the_frame_limit(engine, variant, bins):
     (1,048,304) one counter reaches 65,519: the counts must still be exact.
     One frame more and the engine must not pick a packed variant.  Variant 31
     pairs rows in a word; 45 (50 bins) and 49 (30 bins) interleave the halves
-    by row parity."""
+    by row parity, and 53 and 51 are the multi-tile kernel on that layout."""
     pa = _pa()
     n_max = (65535 - 16) * 16
     nD = 3
