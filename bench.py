@@ -78,38 +78,66 @@ def lds_peak():
         return dict(LDS_PEAK_FALLBACK, source="recorded lds_peak.x figures")
 
 
-def cpu_baseline(atoms_sample, frames, bins, nthreads):
-    """Reference PARENT.x (built from its sources, -O3) timed on the host on a
-    bounded sample: a chain of `atoms_sample` atoms with the same frame count
-    and distributions.  Falls back to the oracle restatement."""
-    ref = os.path.join(ROOT, "oracle", "_ref", "PARENT_O3.x")
-    port = os.path.join(ROOT, "oracle", "_build", "oracle_parent.x")
+def _time_cpu(exe, atoms_sample, frames, bins, nthreads):
+    """`exe` (a PARENT.x-compatible program) on a synthetic chain of
+    `atoms_sample` atoms: its 'Calculation time' line (PARENT.cpp:498-1489)
+    and the pair-frame updates it covers, or None."""
     import parent_amd
     with tempfile.TemporaryDirectory() as td:
         bat = os.path.join(td, "sample.bat")
         parent_amd.synth_write_bat(bat, atoms_sample, frames, 12345)
         env = dict(os.environ, OMP_NUM_THREADS=str(nthreads))
         env["LD_LIBRARY_PATH"] = "/opt/conda/lib:" + env.get("LD_LIBRARY_PATH", "")
-        for kind, exe in (("reference", ref), ("port", port)):
-            if not os.path.exists(exe):
-                continue
-            try:
-                r = subprocess.run([exe, bat, os.path.join(td, "o.par")] + [str(bins)] * 6,
-                                   capture_output=True, text=True, timeout=600, env=env)
-            except subprocess.TimeoutExpired:
-                continue
-            calc = [ln for ln in r.stdout.splitlines() if ln.startswith("Calculation time:")]
-            if r.returncode != 0 or not calc:
-                log(f"cpu baseline {kind} failed: rc={r.returncode} {r.stderr[-300:]}")
-                continue
-            secs = float(calc[-1].split(":")[1])
-            D = 3 * atoms_sample - 6
-            upd = D * (D - 1) / 2 * frames
-            return {"value": upd / secs, "unit": "pair-frame updates/s", "cores": nthreads, "kind": kind,
-                    "sample": f"{atoms_sample}-atom chain ({D} dofs, {D * (D - 1) // 2} pairs) x {frames} "
-                              f"frames, {bins} bins, same distributions; 'Calculation time' span "
-                              f"(PARENT.cpp:498-1489) = {secs:.2f} s; build: -O3 -march=x86-64-v3 -fopenmp"}
+        try:
+            r = subprocess.run([exe, bat, os.path.join(td, "o.par")] + [str(bins)] * 6,
+                               capture_output=True, text=True, timeout=600, env=env)
+        except subprocess.TimeoutExpired:
+            return None
+        calc = [ln for ln in r.stdout.splitlines() if ln.startswith("Calculation time:")]
+        if r.returncode != 0 or not calc:
+            log(f"cpu run of {os.path.basename(exe)} failed: rc={r.returncode} {r.stderr[-300:]}")
+            return None
+        D = 3 * atoms_sample - 6
+        return float(calc[-1].split(":")[1]), D * (D - 1) / 2 * frames
+
+
+def cpu_baseline(atoms_sample, frames, bins, nthreads):
+    """Reference PARENT.x (built from its sources, -O3) timed on the host on a
+    bounded sample: a chain of `atoms_sample` atoms with the same frame count
+    and distributions.  Falls back to the oracle restatement."""
+    ref = os.path.join(ROOT, "oracle", "_ref", "PARENT_O3.x")
+    port = os.path.join(ROOT, "oracle", "_build", "oracle_parent.x")
+    for kind, exe in (("reference", ref), ("port", port)):
+        if not os.path.exists(exe):
+            continue
+        t = _time_cpu(exe, atoms_sample, frames, bins, nthreads)
+        if t is None:
+            continue
+        secs, upd = t
+        D = 3 * atoms_sample - 6
+        return {"value": upd / secs, "unit": "pair-frame updates/s", "cores": nthreads, "kind": kind,
+                "sample": f"{atoms_sample}-atom chain ({D} dofs, {D * (D - 1) // 2} pairs) x {frames} "
+                          f"frames, {bins} bins, same distributions; 'Calculation time' span "
+                          f"(PARENT.cpp:498-1489) = {secs:.2f} s; build: "
+                          + ("-O3 -march=x86-64-v3 -fopenmp" if kind == "reference" else "oracle restatement")}
     return None
+
+
+def cpu_shipped_flags(atoms_sample, frames, bins, nthreads):
+    """The reference PARENT.x built with its shipped flags (-fopenmp -Wall,
+    i.e. -O0), on a smaller bounded sample: the slower of the two reference
+    builds, reported beside the -O3 baseline."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "PARENT.x")
+    if atoms_sample <= 0 or not os.path.exists(exe):
+        return None
+    t = _time_cpu(exe, atoms_sample, frames, bins, nthreads)
+    if t is None:
+        return None
+    secs, upd = t
+    D = 3 * atoms_sample - 6
+    return {"value": upd / secs, "unit": "pair-frame updates/s", "cores": nthreads, "kind": "reference",
+            "sample": f"{atoms_sample}-atom chain ({D} dofs) x {frames} frames, {bins} bins; 'Calculation "
+                      f"time' = {secs:.2f} s; build: the reference's shipped flags (-fopenmp -Wall, -O0)"}
 
 
 def roofline_of(pa, method, bins, updates, kernel_ms, workload, D, P, frames):
@@ -221,9 +249,11 @@ def run_single(args, pa, nthreads):
 
     roofline = roofline_of(pa, st["method"], bins, updates, pair_avg, args.workload, D, P, frames)
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_sample_atoms, frames, bins, nthreads)
+    shipped = None if args.no_cpu else cpu_shipped_flags(args.cpu_shipped_atoms, frames, bins, nthreads)
     extra = {"phases": {"prepare_ms": sum(prep_ms) / len(prep_ms), "pairs_ms": pair_avg,
                         "pairs_kernel_updates_per_s": kernel_rate},
-             "plain": plain, "mist": mist, "roofline": roofline, "cpu_baseline": cpu}
+             "plain": plain, "mist": mist, "roofline": roofline, "cpu_baseline": cpu,
+             "cpu_shipped_flags": shipped}
     if tune is not None:
         extra["tuning"] = tune
     eng.close()
@@ -403,6 +433,8 @@ def main():
     ap.add_argument("--method", default="auto", choices=["auto", "plain", "atomic", "mfma"])
     ap.add_argument("--plain-steps", type=int, default=1, help="timed steps of the plain kernel (0: skip)")
     ap.add_argument("--cpu-sample-atoms", type=int, default=500)
+    ap.add_argument("--cpu-shipped-atoms", type=int, default=200,
+                    help="sample for the reference built with its shipped flags (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mist", action="store_true")
     ap.add_argument("--tune-report", action="store_true", help="print the autotuner's table")
