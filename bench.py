@@ -9,7 +9,7 @@ is uploaded to HBM before the timed region.
 Workloads (--workload):
   protein_1500x100k_b50   default: 1,500-atom chain, 4,494 dofs (10,095,771 pairs)
                           x 100,000 frames, 50 bins for every type
-  protein_1500x100k_bB    same at B = 5, 10, 20, 30 bins
+  protein_1500x100k_bB    same at B = 5, 10, 20, 30, 75, 100 bins
   chain_100x10k_b50       100-atom chain, 294 dofs x 10,000 frames (the small case)
   large_5000x200k_b30     5,000-atom chain, 14,994 dofs (112,402,521 pairs) x 200,000 frames, 30 bins
   sweep_1500x100k         the default trajectory at B in {5,10,20,30,40,50,75,100}: one step = one
@@ -45,6 +45,8 @@ WORKLOADS = {
     "protein_1500x100k_b10": (1500, 100_000, 10),
     "protein_1500x100k_b20": (1500, 100_000, 20),
     "protein_1500x100k_b30": (1500, 100_000, 30),
+    "protein_1500x100k_b75": (1500, 100_000, 75),
+    "protein_1500x100k_b100": (1500, 100_000, 100),
     "chain_100x10k_b50": (100, 10_000, 50),
     "large_5000x200k_b30": (5000, 200_000, 30),
 }

@@ -70,7 +70,8 @@ int atomic_config_count();
 AtomicConfig atomic_config(int id);
 // 0 plain LDS histograms, 1 sliced (bank-conflict-free), 2 packed sliced,
 // 3 sliced with the perm + dot2 fill, 4 interleaved packed with that fill,
-// 5 and 6 the layout of 4 in k_pairs_multi (2 and 1 tiles per workgroup)
+// 5 and 6 the layout of 4 in k_pairs_multi (2 and 1 tiles per workgroup),
+// 7 k_pairs_multi with 4 sub-counters per cell
 int atomic_kind(int id);
 size_t atomic_lds_bytes(int id);
 // Packed sliced histograms keep 16-bit counters of the frames of one lane

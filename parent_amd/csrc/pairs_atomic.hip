@@ -753,7 +753,11 @@ __global__ __launch_bounds__(BLOCK, 1) void k_pairs_sliced(PairInputs in0, TileQ
 // ---------------------------------------------------------------------------
 // A workgroup takes TPW consecutive tiles of its XCD's range, with the
 // interleaved (PACK = 2) or 32-bit (PACK = 1) layout of the DOT sliced
-// kernel.  Per tile: fill; barrier; an epilogue that zeroes each group it
+// kernel, with SL sub-counters per cell: 16 (conflict-free) or 4.  With 4,
+// the 4 lanes of a 16-lane LDS cycle that share a sub-counter index meet in
+// a bank only when their cells fall on the same 8-group residue, so a fill
+// costs about 1.7 cycles per 16 lanes instead of about 3 for unsliced
+// histograms, at a quarter of the LDS: 100 x 100 bins in 80,800 bytes.  Per tile: fill; barrier; an epilogue that zeroes each group it
 // reads (the next tile needs no clear); the next tile's queue entry and first
 // index words requested while this tile's table loads are in flight;
 // barrier; the tile's cross-wave sum on wave 0 while the other waves start
@@ -763,13 +767,15 @@ __global__ __launch_bounds__(BLOCK, 1) void k_pairs_sliced(PairInputs in0, TileQ
 // (the tile loop unrolled, the kernel-argument inputs copied per tile) ran
 // at half the rate, so the loop stays rolled and one trajectory's inputs are
 // read from the argument in place.
-template <int BLOCK, int SLOTC, int PACK, int TPW, bool BATCH>
+template <int BLOCK, int SLOTC, int PACK, int TPW, bool BATCH, int SL = 16>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(BLOCK / 128))) void k_pairs_multi(
     PairInputs in0, TileQueue q, int t_begin, int t_end, const int* __restrict__ dump_begin,
     const int* __restrict__ dump_entries, int* __restrict__ dump_out) {
     extern __shared__ unsigned hist[];
     constexpr int kWaves = BLOCK / 64;
-    constexpr int kSlotW = SLOTC * 16;                   // 16-word groups (PACK 2) or cells (PACK 1)
+    static_assert(SL == 16 || SL == 4, "sub-counters per cell");
+    constexpr int kSlotW = SLOTC * SL;                   // SL-word groups (PACK 2) or cells (PACK 1)
+    constexpr int NQ = SL / 4;                           // uint4 per group
     constexpr int kMaxG = (SLOTC + BLOCK - 1) / BLOCK;   // groups per thread
     constexpr int per_iter = BLOCK * 16;
     __shared__ double sT[2][kWaves];
@@ -779,15 +785,15 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(BLOCK / 1
     auto has = [&](int local) { return local < chunk && tbase + local < t_end; };
     if (!has(l0)) return;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const unsigned s16 = threadIdx.x & 15;
+    const unsigned s16 = threadIdx.x & (SL - 1);  // the lane's sub-counter
     const int toff = (int)threadIdx.x * 16;
     const unsigned acc = (unsigned)(uintptr_t)(lds_u32*)hist + s16 * 4u;
     uint4* h4 = reinterpret_cast<uint4*>(hist);
     unsigned one = 1u;
     asm("" : "+v"(one));
-    int quad[4];
+    int quad[NQ];
 #pragma unroll
-    for (int k = 0; k < 4; k++) quad[k] = (k + (threadIdx.x >> 1)) & 3;
+    for (int k = 0; k < NQ; k++) quad[k] = (k + (threadIdx.x >> 1)) & (NQ - 1);
 
     // the queue and the batch descriptors are not written while the kernel
     // runs: read through the constant address space they are scalar loads
@@ -819,8 +825,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(BLOCK / 1
         }
     };
     auto cell_word = [&](unsigned a, unsigned b, unsigned Bj) {
-        if constexpr (PACK == 1) return ((__umul24(a, Bj) + b) << 4) + s16;
-        else return (__umul24(a, (Bj + 1u) & ~1u) << 3) + (b << 4) + s16;
+        if constexpr (PACK == 1) return (__umul24(a, Bj) + b) * SL + s16;
+        else return __umul24(a, (Bj + 1u) & ~1u) * (SL / 2) + b * SL + s16;
     };
     auto cell_inc = [&](unsigned a) {
         if constexpr (PACK == 1) return 1u;
@@ -856,8 +862,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(BLOCK / 1
             const uint8_t* rowp = in.bins + (size_t)x.gi * in.Np;
             const uint8_t* colp = in.bins + (size_t)x.gj * in.Np;
             const __amdgpu_buffer_rsrc_t rr = index_row(rowp, in.Np), cr = index_row(colp, in.Np);
-            const unsigned rowmul = PACK == 1 ? (unsigned)Bj * 64u : (((unsigned)Bj + 1u) & ~1u) * 32u;
-            const unsigned kdot = rowmul | (64u << 16);
+            // byte offsets: row a * rowmul, column b * 4 SL
+            const unsigned rowmul = PACK == 1 ? (unsigned)Bj * (4u * SL) : (((unsigned)Bj + 1u) & ~1u) * (2u * SL);
+            const unsigned kdot = rowmul | ((4u * SL) << 16);
             auto process = [&](const uint4& R, const uint4& C) {
 #pragma unroll
                 for (int kk = 0; kk < 4; kk++) {
@@ -925,7 +932,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(BLOCK / 1
                     const unsigned m = PACK == 2 ? 0xFFFFu : 0xFFFFFFFFu;
                     unsigned n = 0;
 #pragma unroll
-                    for (int kk = 0; kk < 4; kk++) {
+                    for (int kk = 0; kk < NQ; kk++) {
                         const uint4 v = g4[kk];
                         n += ((v.x >> sh) & m) + ((v.y >> sh) & m) + ((v.z >> sh) & m) + ((v.w >> sh) & m);
                     }
@@ -943,9 +950,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(BLOCK / 1
             unsigned lo = 0, hi = 0;
             if (g < ngroups) {
 #pragma unroll
-                for (int q4 = 0; q4 < 4; q4++) {
-                    const uint4 v = h4[g * 4 + quad[q4]];
-                    if (more) h4[g * 4 + quad[q4]] = make_uint4(0, 0, 0, 0);
+                for (int q4 = 0; q4 < NQ; q4++) {
+                    const uint4 v = h4[g * NQ + quad[q4]];
+                    if (more) h4[g * NQ + quad[q4]] = make_uint4(0, 0, 0, 0);
                     if constexpr (PACK == 1) {
                         lo += (v.x + v.y) + (v.z + v.w);
                     } else {
@@ -1003,7 +1010,8 @@ namespace {
 // groups of 16 words, (B - 1) * S / 2 + B for B x B cells with S = B rounded
 // up to even (1275 -> B <= 50, 820 -> B <= 40, 465 -> B <= 30); kinds 5
 // and 6: the layout of kind 4 in k_pairs_multi, 2 consecutive tiles per
-// workgroup (5) or 1 (6).
+// workgroup (5) or 1 (6); kind 7: k_pairs_multi with 4 sub-counters per
+// cell, slot size in groups of 4 words (5050 -> B <= 100, 2887 -> B <= 75).
 #define PARENT_ATOMIC_VARIANTS(X)     \
     X(0, 0, 1, 1, 256, 2560)          \
     X(1, 0, 2, 2, 256, 2560)          \
@@ -1060,7 +1068,10 @@ namespace {
     X(52, 5, 1, 1, 256, 1275)         \
     X(53, 6, 1, 1, 1024, 1275)        \
     X(54, 6, 1, 1, 1024, 820)         \
-    X(55, 6, 1, 1, 256, 465)
+    X(55, 6, 1, 1, 256, 465)          \
+    X(56, 7, 1, 1, 1024, 5050)        \
+    X(57, 7, 1, 1, 1024, 2887)        \
+    X(58, 7, 1, 1, 512, 2887)
 
 struct AtomicVariant {
     AtomicConfig cfg;
@@ -1075,15 +1086,16 @@ hipError_t launch_variant(dim3 grid, size_t lds, hipStream_t s, const PairInputs
                           int t1, const int* db, const int* de, int* dout) {
     if constexpr (KIND >= 5) {
         constexpr int TPW = KIND == 5 ? 2 : 1;
-        const void* mf = q.descs ? reinterpret_cast<const void*>(&k_pairs_multi<BLK, SL, 2, TPW, true>)
-                                 : reinterpret_cast<const void*>(&k_pairs_multi<BLK, SL, 2, TPW, false>);
+        constexpr int NS = KIND == 7 ? 4 : 16;
+        const void* mf = q.descs ? reinterpret_cast<const void*>(&k_pairs_multi<BLK, SL, 2, TPW, true, NS>)
+                                 : reinterpret_cast<const void*>(&k_pairs_multi<BLK, SL, 2, TPW, false, NS>);
         hipError_t e = hipFuncSetAttribute(mf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
         // TPW consecutive tiles of an XCD's range per workgroup
         const int chunk = (t1 - t0 + 7) / 8;
         const dim3 g((unsigned)((chunk + TPW - 1) / TPW) * 8);
-        if (q.descs) k_pairs_multi<BLK, SL, 2, TPW, true><<<g, BLK, lds, s>>>(in, q, t0, t1, db, de, dout);
-        else k_pairs_multi<BLK, SL, 2, TPW, false><<<g, BLK, lds, s>>>(in, q, t0, t1, db, de, dout);
+        if (q.descs) k_pairs_multi<BLK, SL, 2, TPW, true, NS><<<g, BLK, lds, s>>>(in, q, t0, t1, db, de, dout);
+        else k_pairs_multi<BLK, SL, 2, TPW, false, NS><<<g, BLK, lds, s>>>(in, q, t0, t1, db, de, dout);
         return hipGetLastError();
     }
     constexpr int PACK = KIND == 2 || KIND == 4 ? 2 : 1;
@@ -1120,9 +1132,10 @@ AtomicConfig atomic_config(int id) { return kAtomicVariants[id].cfg; }
 int atomic_kind(int id) { return kAtomicVariants[id].kind; }
 size_t atomic_lds_bytes(int id) {
     const AtomicVariant& v = kAtomicVariants[id];
-    const size_t per_slot = v.kind == 0 ? (size_t)v.slot * 4
+    const size_t per_slot = v.kind == 0   ? (size_t)v.slot * 4
                             : v.kind == 2 ? (size_t)v.slot / 2 * 64
-                                          : (size_t)v.slot * 64;  // cells (1, 3) or 16-word groups (4, 5)
+                            : v.kind == 7 ? (size_t)v.slot * 16    // 4-word groups
+                                          : (size_t)v.slot * 64;   // cells (1, 3) or 16-word groups (4-6)
     return (size_t)v.cfg.tr * v.cfg.tc * per_slot;
 }
 bool atomic_fits(int id, int bmax, size_t lds_max, int n_frames) {
@@ -1132,7 +1145,9 @@ bool atomic_fits(int id, int bmax, size_t lds_max, int n_frames) {
     const long long cells = v.kind == 2  ? (long long)bmax * even
                             : v.kind >= 4 ? (long long)(bmax - 1) * (even / 2) + bmax
                                                          : (long long)bmax * bmax;
-    if (halves && n_frames > kPackedMaxFrames) return false;
+    // a 16-bit half counts the frames of one sub-counter index: N/16 + 16
+    // with 16 of them, N/4 + 16 with 4
+    if (halves && n_frames > (v.kind == 7 ? (65535 - 16) * 4 : kPackedMaxFrames)) return false;
     return cells <= v.slot && atomic_lds_bytes(id) <= lds_max;
 }
 

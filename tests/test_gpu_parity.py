@@ -226,17 +226,19 @@ def test_mfma_out_of_range_falls_back(engine):
     assert engine.stats()["method"] == pa.METHOD_ATOMIC
 
 
-@pytest.mark.parametrize("variant,bins", [(31, 50), (45, 50), (49, 30), (53, 50), (51, 30)])
-def test_packed_counters_at_the_frame_limit(engine, variant, bins):
+@pytest.mark.parametrize("variant,bins,slices", [(31, 50, 16), (45, 50, 16), (49, 30, 16), (53, 50, 16),
+                                                 (51, 30, 16), (56, 100, 4), (57, 75, 4)])
+def test_packed_counters_at_the_frame_limit(engine, variant, bins, slices):
     """The packed sliced histograms keep 16-bit counters of the frames of one
     lane position.  A constant trajectory puts every frame of every pair in
     one cell, so at the largest frame count the packed variants accept
     (1,048,304) one counter reaches 65,519: the counts must still be exact.
     One frame more and the engine must not pick a packed variant.  Variant 31
     pairs rows in a word; 45 (50 bins) and 49 (30 bins) interleave the halves
-    by row parity, and 53 and 51 are the multi-tile kernel on that layout."""
+    by row parity, and 53 and 51 are the multi-tile kernel on that layout; 56
+    and 57 keep 4 sub-counters per cell, so their limit is (65535 - 16) * 4."""
     pa = _pa()
-    n_max = (65535 - 16) * 16
+    n_max = (65535 - 16) * slices
     nD = 3
     D = 3 * nD + 3
     row = np.array([0.12] * (nD + 2) + [1.9] * (nD + 1) + [3.0] * nD, dtype=np.float32)
