@@ -210,8 +210,14 @@ def run_single(args, pa, nthreads):
         for t in tune:
             log(f"tune: method {t['method']} variant {t['variant']} tile {t['tile_rows']}x{t['tile_cols']} "
                 f"block {t['block']}: {t['ms']:.3f} ms, {t['updates_per_s']:.3e} updates/s")
-    for _ in range(args.warmup):
+    first_ms = None
+    for w in range(args.warmup):
+        t = time.perf_counter()
         step(method)
+        eng.sync()
+        if w == 0:  # the first AUTO step also runs the autotuner
+            first_ms = 1e3 * (time.perf_counter() - t)
+            log(f"first step (with tuning under auto): {first_ms:.1f} ms")
     eng.sync()
     times, prep_ms, pair_ms = [], [], []
     for _ in range(args.steps):
@@ -253,7 +259,7 @@ def run_single(args, pa, nthreads):
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_sample_atoms, frames, bins, nthreads)
     shipped = None if args.no_cpu else cpu_shipped_flags(args.cpu_shipped_atoms, frames, bins, nthreads)
     extra = {"phases": {"prepare_ms": sum(prep_ms) / len(prep_ms), "pairs_ms": pair_avg,
-                        "pairs_kernel_updates_per_s": kernel_rate},
+                        "pairs_kernel_updates_per_s": kernel_rate, "first_step_ms": first_ms},
              "plain": plain, "mist": mist, "roofline": roofline, "cpu_baseline": cpu,
              "cpu_shipped_flags": shipped}
     if tune is not None:

@@ -411,10 +411,11 @@ std::vector<int> tune_work(int D, int tr, int tc, const std::vector<std::pair<in
     return t;
 }
 
-// Times `launch` twice (first run warms caches) with events on the engine stream.
+// Times `launch` `reps` times with events on the engine stream and keeps the
+// last timing (the first run loads the code object and warms the caches).
 template <class F>
-int time_launch(parent_engine* e, F launch, double* ms) {
-    for (int rep = 0; rep < 2; rep++) {
+int time_launch(parent_engine* e, F launch, double* ms, int reps = 2) {
+    for (int rep = 0; rep < reps; rep++) {
         HIPCHK(e, hipEventRecord(e->ev[4], e->stream));
         HIPCHK(e, launch());
         HIPCHK(e, hipEventRecord(e->ev[5], e->stream));
@@ -424,25 +425,28 @@ int time_launch(parent_engine* e, F launch, double* ms) {
     return PARENT_OK;
 }
 
+// Two stages: every fitting configuration is timed on the middle run of
+// tune_rows (an eighth of the sample), then the kTuneFinalists fastest are
+// timed again on all of it and the fastest of those is kept.  The table holds
+// one row per configuration, the finalists' rows from the second stage.
+constexpr int kTuneFinalists = 4;
+
 int run_autotune(parent_engine* e, const Traj& tj, BinSet& s) {
     e->last_tune.clear();
     const int D = tj.D;
     const auto rows = tune_rows(D);
     const PairInputs in = with_nlogn(e, pair_inputs(tj, s));
     DevBuf dt;
-    int best_m = PARENT_METHOD_PLAIN, best_v = 0;
-    double best_rate = -1;
     // candidates: every fitting atomic configuration, then the MFMA kernel
     std::vector<std::pair<int, int>> cand;
     for (int id = 0; id < atomic_config_count(); id++)
         if (variant_fits(e, s, id, tj.N)) cand.push_back({PARENT_METHOD_ATOMIC, id});
     if (mfma_fits(e, s)) cand.push_back({PARENT_METHOD_MFMA, 0});
-    for (auto& mv : cand) {
-        const int m = mv.first, id = mv.second;
+    auto measure = [&](int m, int id, const std::vector<std::pair<int, int>>& rr, parent_tune_entry* te) {
         int tr, tc;
         tile_shape(s, m, id, &tr, &tc);
         const int block = m == PARENT_METHOD_MFMA ? 256 : atomic_config(id).block;
-        std::vector<int> t = tune_work(D, tr, tc, rows);
+        std::vector<int> t = tune_work(D, tr, tc, rr);
         const double updates = (double)tile_pairs(D, t, tr, tc) * tj.N;
         HIPCHK(e, dt.ensure(std::max<size_t>(t.size(), 2) * sizeof(int)));
         HIPCHK(e, hipMemcpyAsync(dt.p, t.data(), t.size() * sizeof(int), hipMemcpyHostToDevice, e->stream));
@@ -450,12 +454,37 @@ int run_autotune(parent_engine* e, const Traj& tj, BinSet& s) {
         const TileQueue q{dt.as<int>(), (int)(t.size() / 2), 2, nullptr};
         if (int rc = time_launch(e, [&] { return launch_tiled(e, in, bmax2d(s), m, id, q, nullptr, nullptr, nullptr); }, &ms))
             return rc;
-        parent_tune_entry te{m, id, tr, tc, block, ms, updates, updates / (ms * 1e-3)};
+        *te = parent_tune_entry{m, id, tr, tc, block, ms, updates, updates / (ms * 1e-3)};
+        return PARENT_OK;
+    };
+    const std::vector<std::pair<int, int>> probe{rows[rows.size() / 2]};
+    const bool two_stage = rows.size() > 1 && (int)cand.size() > kTuneFinalists;
+    for (auto& mv : cand) {
+        parent_tune_entry te;
+        if (int rc = measure(mv.first, mv.second, two_stage ? probe : rows, &te)) return rc;
         e->last_tune.push_back(te);
+    }
+    // the contenders: every row, or the finalists re-timed on the full sample
+    std::vector<int> order(e->last_tune.size());
+    for (size_t k = 0; k < order.size(); k++) order[k] = (int)k;
+    if (two_stage) {
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+            return e->last_tune[x].updates_per_s > e->last_tune[y].updates_per_s;
+        });
+        order.resize(kTuneFinalists);
+        for (int k : order) {
+            parent_tune_entry& te = e->last_tune[k];
+            if (int rc = measure(te.method, te.variant, rows, &te)) return rc;
+        }
+    }
+    int best_m = PARENT_METHOD_PLAIN, best_v = 0;
+    double best_rate = -1;
+    for (int k : order) {
+        const parent_tune_entry& te = e->last_tune[k];
         if (te.updates_per_s > best_rate) {
             best_rate = te.updates_per_s;
-            best_m = m;
-            best_v = id;
+            best_m = te.method;
+            best_v = te.variant;
         }
     }
     e->tuned.push_back({s.bins[3], s.bins[4], s.bins[5], tj.N, D, best_m, best_v});
