@@ -248,11 +248,13 @@ int parent_engine_mist(parent_engine* eng, int32_t* edges, double* mi, int32_t* 
 int parent_engine_set_entropies(parent_engine* eng, int n_dihedrals, const double* s1d,
                                 const double* s2d);
 
-/* Times every pair-kernel configuration that fits the device on one run of 12
- * row dofs against all their partners, re-times the 4 fastest on the whole
- * sample (8 such runs spread over the dofs) and keeps the fastest of those for
+/* Times every pair-kernel configuration that fits the device on one run of
+ * row dofs against all their partners (12 rows, more when the configuration's
+ * tiles are large, so that the sample holds 16 tiles per compute unit),
+ * re-times the 3 fastest on 32 such runs spread over the dofs (at least 32
+ * tiles per compute unit) and keeps the fastest of those for
  * PARENT_METHOD_AUTO on this (bins, frames, dofs).  Rows of the table carry
- * their own sample's updates; the 4 finalists' rows are the whole-sample ones.
+ * their own sample's updates; the 3 finalists' rows are the 32-run ones.
  * PARENT_METHOD_AUTO tunes on first use by itself; this call re-tunes and
  * returns the table (up to max_entries rows; *n_entries = rows measured).
  * No counterpart in the reference (its single loop nest has no variants). */

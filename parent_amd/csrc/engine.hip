@@ -111,6 +111,7 @@ struct parent_engine {
     hipEvent_t ev[8] = {};
     std::string err;
     int lds_max = 65536;
+    int cus = 256;  // compute units, sizes the autotuner's samples
 
     Traj traj;
     Scratch scratch;
@@ -392,15 +393,17 @@ int choose_atomic_config(const parent_engine* e, const BinSet& s, int n_frames) 
     return -1;
 }
 
-// Rows sampled by the autotuner: 8 runs of 12 rows spread over the dof range.
-// Each configuration's rate is its own work list's pair count (tile heights
-// that do not divide 12 cover a few more rows) over its time.
-std::vector<std::pair<int, int>> tune_rows(int D) {
+// Rows sampled by the autotuner: `runs` runs of `height` rows (a multiple of
+// 12) spread over the dof range, each starting on a multiple of 12.  Each
+// configuration's rate is its own work list's pair count (tile heights that
+// do not divide 12 cover a few more rows) over its time.
+std::vector<std::pair<int, int>> tune_rows(int D, int runs = 8, int height = 12) {
     std::vector<std::pair<int, int>> rows;
-    for (int k = 0; k < 8; k++) {
-        int r = (int)((k + 0.5) * (D - 1) / 8.0) / 12 * 12;
-        int r1 = std::min(r + 12, D - 1);
-        if (r < r1 && (rows.empty() || rows.back().second <= r)) rows.push_back({r, r1});
+    for (int k = 0; k < runs; k++) {
+        int r = (int)((k + 0.5) * (D - 1) / runs) / 12 * 12;
+        if (!rows.empty()) r = std::max(r, rows.back().second);
+        int r1 = std::min(r + height, D - 1);
+        if (r < r1) rows.push_back({r, r1});
     }
     return rows;
 }
@@ -411,30 +414,46 @@ std::vector<int> tune_work(int D, int tr, int tc, const std::vector<std::pair<in
     return t;
 }
 
+// A tuning work list of `runs` runs for tiles of tr x tc dofs, the runs made
+// taller (12, 24, 48, ... rows) until the list holds at least min_tiles tiles
+// or covers every row: a configuration with large tiles (the MFMA kernel at a
+// few bins: 25 x 25 dofs) otherwise gets a fraction of a wave of workgroups
+// and is timed far below its full-size rate.
+std::vector<int> tune_sample(int D, int tr, int tc, int runs, size_t min_tiles) {
+    for (int h = 12;; h *= 2) {
+        std::vector<int> t = tune_work(D, tr, tc, tune_rows(D, runs, h));
+        if (t.size() / 2 >= min_tiles || (size_t)h * runs >= (size_t)D) return t;
+    }
+}
+
 // Times `launch` `reps` times with events on the engine stream and keeps the
-// last timing (the first run loads the code object and warms the caches).
+// fastest run after the first (which warms the caches).
 template <class F>
 int time_launch(parent_engine* e, F launch, double* ms, int reps = 2) {
+    *ms = 0;
     for (int rep = 0; rep < reps; rep++) {
         HIPCHK(e, hipEventRecord(e->ev[4], e->stream));
         HIPCHK(e, launch());
         HIPCHK(e, hipEventRecord(e->ev[5], e->stream));
         HIPCHK(e, hipStreamSynchronize(e->stream));
+        const double t = elapsed(e->ev[4], e->ev[5]);
+        if (rep == 1 || (rep > 1 && t < *ms)) *ms = t;
     }
-    *ms = elapsed(e->ev[4], e->ev[5]);
     return PARENT_OK;
 }
 
-// Two stages: every fitting configuration is timed on the middle run of
-// tune_rows (an eighth of the sample), then the kTuneFinalists fastest are
-// timed again on all of it and the fastest of those is kept.  The table holds
-// one row per configuration, the finalists' rows from the second stage.
-constexpr int kTuneFinalists = 4;
+// Two stages: every fitting configuration is timed on one run in the middle
+// of the dof range, then the kTuneFinalists fastest are timed again, best of
+// kFinalReps, on kFinalRuns runs spread over it, and the fastest of those is
+// kept.  Samples hold at least kProbeTilesPerCu / kFinalTilesPerCu tiles per
+// compute unit.  The table holds one row per configuration, the finalists'
+// rows from the second stage.
+constexpr int kTuneFinalists = 3, kFinalRuns = 32, kFinalReps = 3;
+constexpr int kProbeTilesPerCu = 16, kFinalTilesPerCu = 32;
 
 int run_autotune(parent_engine* e, const Traj& tj, BinSet& s) {
     e->last_tune.clear();
     const int D = tj.D;
-    const auto rows = tune_rows(D);
     const PairInputs in = with_nlogn(e, pair_inputs(tj, s));
     DevBuf dt;
     // candidates: every fitting atomic configuration, then the MFMA kernel
@@ -442,29 +461,30 @@ int run_autotune(parent_engine* e, const Traj& tj, BinSet& s) {
     for (int id = 0; id < atomic_config_count(); id++)
         if (variant_fits(e, s, id, tj.N)) cand.push_back({PARENT_METHOD_ATOMIC, id});
     if (mfma_fits(e, s)) cand.push_back({PARENT_METHOD_MFMA, 0});
-    auto measure = [&](int m, int id, const std::vector<std::pair<int, int>>& rr, parent_tune_entry* te) {
+    auto measure = [&](int m, int id, int runs, int tiles_per_cu, int reps, parent_tune_entry* te) -> int {
         int tr, tc;
         tile_shape(s, m, id, &tr, &tc);
         const int block = m == PARENT_METHOD_MFMA ? 256 : atomic_config(id).block;
-        std::vector<int> t = tune_work(D, tr, tc, rr);
+        std::vector<int> t = tune_sample(D, tr, tc, runs, (size_t)tiles_per_cu * e->cus);
         const double updates = (double)tile_pairs(D, t, tr, tc) * tj.N;
         HIPCHK(e, dt.ensure(std::max<size_t>(t.size(), 2) * sizeof(int)));
         HIPCHK(e, hipMemcpyAsync(dt.p, t.data(), t.size() * sizeof(int), hipMemcpyHostToDevice, e->stream));
         double ms = 0;
         const TileQueue q{dt.as<int>(), (int)(t.size() / 2), 2, nullptr};
-        if (int rc = time_launch(e, [&] { return launch_tiled(e, in, bmax2d(s), m, id, q, nullptr, nullptr, nullptr); }, &ms))
+        if (int rc = time_launch(e, [&] { return launch_tiled(e, in, bmax2d(s), m, id, q, nullptr, nullptr, nullptr); }, &ms, reps))
             return rc;
         *te = parent_tune_entry{m, id, tr, tc, block, ms, updates, updates / (ms * 1e-3)};
         return PARENT_OK;
     };
-    const std::vector<std::pair<int, int>> probe{rows[rows.size() / 2]};
-    const bool two_stage = rows.size() > 1 && (int)cand.size() > kTuneFinalists;
+    const bool two_stage = (int)cand.size() > kTuneFinalists;
     for (auto& mv : cand) {
         parent_tune_entry te;
-        if (int rc = measure(mv.first, mv.second, two_stage ? probe : rows, &te)) return rc;
+        const int rc = two_stage ? measure(mv.first, mv.second, 1, kProbeTilesPerCu, 2, &te)
+                                 : measure(mv.first, mv.second, kFinalRuns, kFinalTilesPerCu, kFinalReps, &te);
+        if (rc) return rc;
         e->last_tune.push_back(te);
     }
-    // the contenders: every row, or the finalists re-timed on the full sample
+    // the contenders: every row, or the finalists re-timed on the larger sample
     std::vector<int> order(e->last_tune.size());
     for (size_t k = 0; k < order.size(); k++) order[k] = (int)k;
     if (two_stage) {
@@ -474,7 +494,7 @@ int run_autotune(parent_engine* e, const Traj& tj, BinSet& s) {
         order.resize(kTuneFinalists);
         for (int k : order) {
             parent_tune_entry& te = e->last_tune[k];
-            if (int rc = measure(te.method, te.variant, rows, &te)) return rc;
+            if (int rc = measure(te.method, te.variant, kFinalRuns, kFinalTilesPerCu, kFinalReps, &te)) return rc;
         }
     }
     int best_m = PARENT_METHOD_PLAIN, best_v = 0;
@@ -768,6 +788,9 @@ int parent_engine_create(int device, parent_engine** out) {
     int lds = 0;
     if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
         e->lds_max = lds;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+        e->cus = cus;
     *out = e;
     return PARENT_OK;
 }
