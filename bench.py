@@ -449,7 +449,7 @@ def main():
     ap.add_argument("--variant", type=int, default=-1, help="pin the atomic method's configuration")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (--gpus {args.gpus} requested); "
+        print(f"bench.py: this benchmark runs on one GPU only (--gpus {args.gpus} requested); "
               "the multi-GPU sharding is a design note, not a runnable path.", file=sys.stderr)
         sys.exit(2)
     import parent_amd as pa

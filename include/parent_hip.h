@@ -44,7 +44,7 @@ enum {
     PARENT_E_DATA = 4,        /* a bond length < 0 (PARENT.cpp:522-525) */
     PARENT_E_RANGE = 5,       /* a value falls outside its histogram (reference: out-of-bounds write) */
     PARENT_E_DEVICE = 6,      /* HIP runtime error */
-    PARENT_E_UNSUPPORTED = 7  /* request outside what this build implements (e.g. > 255 bins) */
+    PARENT_E_UNSUPPORTED = 7  /* request outside what this library implements (e.g. > 255 bins) */
 };
 
 /* pair-kernel selection for parent_engine_pairs() */

@@ -39,7 +39,7 @@ int usage(const char* prog) {
 
 int main(int argc, char* argv[]) {
     if (argc == 5 && has(argc, argv, "-b")) {
-        std::cerr << "ERROR: the .bat -> .xtc direction is not provided by this build.\n";
+        std::cerr << "ERROR: the .bat -> .xtc direction is not provided by this program.\n";
         return 1;
     }
     if (argc != 9 && argc != 10) return usage(argv[0]);

@@ -232,7 +232,7 @@ UBQ_DIR = os.path.join(GOLD, "ubq")
 
 def test_oracle_reproduces_ubq_sample_totals(tmp_path):
     """End-to-end known answer of the reference (README.md:108-123): the UBQ_UBM2
-    test system at 50 bins.  The .bat comes from this build's BAT builder (its
+    test system at 50 bins.  The .bat comes from the library's BAT builder (its
     bytes are pinned to the reference builder's in test_cpu_bat_builder.py);
     every TOTAL line of the reference's sample get_PAR_MIE / get_PAR_MIST
     output is reproduced at 12 digits."""
