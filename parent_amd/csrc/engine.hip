@@ -687,8 +687,9 @@ int bfail(parent_batch* b, int code, const std::string& msg) { return fail(b->en
 // Kernel choice for a whole batch.  One configuration runs every tile of the
 // merged queue, and the right one depends on the mix: per pair, a tile costs
 // a fixed part (clearing its LDS histograms, the epilogue) and a part per
-// frame.  Each candidate is timed on the tuning rows of the batch's longest
-// and shortest trajectories; the two timings give the model
+// frame.  Each candidate is timed on a tuning sample (tune_sample: 8 runs of
+// rows) of the batch's longest and shortest trajectories; the two timings
+// give the model
 // ms = alpha * tiles + beta * updates, which predicts the full queue.
 int batch_autotune(parent_engine* e, parent_batch* b, int* method, int* variant) {
     const int n = (int)b->items.size();
@@ -719,7 +720,7 @@ int batch_autotune(parent_engine* e, parent_batch* b, int* method, int* variant)
         for (int p = 0; p < np; p++) {
             const Traj& t = b->items[probes[p]]->traj;
             BinSet& s = b->items[probes[p]]->set;
-            std::vector<int> w = tune_work(t.D, tr, tc, tune_rows(t.D));
+            std::vector<int> w = tune_sample(t.D, tr, tc, 8, (size_t)kProbeTilesPerCu * e->cus);
             if (w.empty()) continue;
             tiles[p] = (double)(w.size() / 2);
             upd[p] = (double)tile_pairs(t.D, w, tr, tc) * t.N;

@@ -350,3 +350,42 @@ def test_bad_inputs_raise(engine):
     with pytest.raises(pa.ParentError) as ei:
         engine.prepare([300] * 6)
     assert ei.value.code == 7
+
+
+@pytest.mark.parametrize("bins,expect_method", [(5, 2), (50, 1)])
+def test_autotune_table_and_choice(bins, expect_method):
+    """The autotuner's table is consistent, AUTO runs the kernel it chose, and
+    each method wins where it is several times faster at full size: the one-hot
+    MFMA kernel at 5 bins (its 25 x 25-dof tiles need a sample grown to fill the
+    GPU, or it is timed on a fraction of a wave and dropped), the LDS-atomic
+    kernels at 50 bins.  Counts of the chosen kernel are checked bit-exact on a
+    pair sample."""
+    pa = _pa()
+    atoms, frames = 1500, 20000
+    print(f"autotune: {atoms} atoms x {frames} frames, {bins} bins", flush=True)
+    vals = pa.synth_values(atoms, frames, 7)
+    with pa.Engine(0) as eng:
+        eng.set_trajectory(vals)
+        eng.prepare([bins] * 6)
+        tune = eng.autotune()
+        assert len(tune) >= 2
+        keys = [(t["method"], t["variant"]) for t in tune]
+        assert len(set(keys)) == len(keys)
+        for t in tune:
+            assert t["method"] in (pa.METHOD_ATOMIC, pa.METHOD_MFMA)
+            assert 0 <= t["variant"] < eng.variant_count(t["method"])
+            assert t["ms"] > 0 and t["updates"] > 0
+            assert t["updates_per_s"] == pytest.approx(t["updates"] / (t["ms"] * 1e-3), rel=1e-9)
+        eng.pairs(pa.METHOD_AUTO)
+        st = eng.stats()
+        assert (st["method"], st["kernel_variant"]) in keys
+        assert st["method"] == expect_method, tune
+        D = 3 * atoms - 6
+        rng = np.random.default_rng(bins)
+        pairs = sorted({tuple(int(x) for x in sorted(rng.choice(D, 2, replace=False))) for _ in range(8)})
+        idx = eng.bins()
+        got = eng.pair_counts(pairs, pa.METHOD_AUTO, [bins] * 3)
+        for (i, j), h in zip(pairs, got):
+            ref = np.zeros((bins, bins), dtype=np.int64)
+            np.add.at(ref, (idx[i].astype(np.int64), idx[j].astype(np.int64)), 1)
+            assert np.array_equal(h.astype(np.int64), ref), (i, j)
