@@ -38,6 +38,31 @@ constexpr int kOperandBytes = 2 * kBuf;     // double-buffered: chunk c+1 is sca
 constexpr int kCountBytes = kTile * kTile * 4;
 constexpr int kLds = kCountBytes > kOperandBytes ? kCountBytes : kOperandBytes;
 constexpr int kMaxUnits = 4;      // (dof, frame group) units per thread: 2 * TD * 8 <= 1024
+constexpr int kPermMaxBins = 12;  // the register-built operands (PERM) serve B <= 12
+constexpr int kIdxStride = kChunk + 16;  // staged index row stride in bytes (PERM)
+constexpr unsigned kPadIndex = 12;       // staged value of padding frames: v_perm selector 12 gives 0x00
+// PERM: two staging buffers of both sides' index rows (at most 64 dofs a
+// side), then the count tile in the same memory
+constexpr int kPermStage = 2 * 2 * 64 * kIdxStride;
+constexpr int kPermLds = kCountBytes > kPermStage ? kCountBytes : kPermStage;
+
+// The 64-bit v_perm_b32 table {hi, lo} whose byte lookup turns a word of 4
+// bin indices into the 4 one-hot bytes (index == k).  v_perm_b32 selector
+// bytes 0-7 pick bytes of {hi, lo} (0-3 from lo), 8-11 replicate the top bit
+// of byte 1, 3, 5 or 7, and 12 gives 0x00.  Bins 0-7 set byte k to 1; bins
+// 8-11 set the top bit of byte 1, 3, 5 or 7 and the caller keeps bit 0 of
+// every byte (& 0x01010101), which the byte-selected 0x80 values do not
+// have.  A row outside the tile (ok false) gets the zero table.
+__device__ __forceinline__ void onehot_table(bool ok, int k, unsigned* hi, unsigned* lo) {
+    *hi = *lo = 0;
+    if (!ok) return;
+    if (k < 4) *lo = 1u << (8 * k);
+    else if (k < 8) *hi = 1u << (8 * (k - 4));
+    else if (k == 8) *lo = 0x8000u;
+    else if (k == 9) *lo = 0x80000000u;
+    else if (k == 10) *hi = 0x8000u;
+    else *hi = 0x80000000u;
+}
 
 // Workgroup barrier that waits for the wave's LDS operations only (global
 // loads in flight are not waited for, unlike __syncthreads()).  The memory
@@ -46,7 +71,14 @@ __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(
 
 }  // namespace
 
-template <bool BATCH>
+// PERM 1 (B <= 8) and 2 (B <= 12): each chunk's bin indices are staged in
+// LDS once, one 16-byte store per (dof, 16 frames), and every lane builds its
+// own MFMA operand row from them with one v_perm_b32 per 4 frames (PERM 2
+// adds a mask).  PERM 0: the one-hot bytes are scattered into LDS operand
+// tiles and cleared again (every B <= 64).  UPT (PERM only): staging units
+// per thread, ceil(2 * TD * 8 / 256); every thread issues all UPT loads, so
+// the waits for a staged chunk count the loads in flight exactly.
+template <bool BATCH, int PERM, int UPT>
 __global__ __launch_bounds__(256, 2) void k_pairs_mfma(PairInputs in0, int B, int TD, TileQueue q,
                                                        int t_begin, int t_end, const int* __restrict__ dump_begin,
                                                        const int* __restrict__ dump_entries, int* __restrict__ dump_out) {
@@ -63,64 +95,6 @@ __global__ __launch_bounds__(256, 2) void k_pairs_mfma(PairInputs in0, int B, in
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int wr = wid >> 1, wc = wid & 1;
 
-    for (int i = tid; i < kOperandBytes / 16; i += 256) reinterpret_cast<uint4*>(smem)[i] = make_uint4(0, 0, 0, 0);
-
-    // Static unit assignment: unit u = (side, dof slot, frame group of 16).
-    // Units of dof slots past the last dof read dof D-1 and fill operand rows
-    // that no epilogue or dump reads; units past the table (u >= nunits, only
-    // in the last partial wave) are masked.
-    const int nunits = 2 * TD * 8;
-    int ubase[kMaxUnits];            // byte offset of the unit's (row block, frame group) origin
-    const uint8_t* usrc[kMaxUnits];  // the dof's bin-index row at the unit's frame group
-    bool uact[kMaxUnits];
-#pragma unroll
-    for (int k = 0; k < kMaxUnits; k++) {
-        const int u = tid + k * 256;
-        const int side = u / (TD * 8), rem = u % (TD * 8), d = rem / 8, g = rem % 8;
-        uact[k] = u < nunits;
-        const int dof = uact[k] ? min((side ? c0 : r0) + d, D - 1) : r0;
-        usrc[k] = in.bins + (size_t)dof * in.Np + g * 16;
-        ubase[k] = uact[k] ? (side ? kTile * kStride : 0) + d * B * kStride + g * 4 : 0;
-    }
-    const int nch = (N + kChunk - 1) / kChunk;  // chunks holding real frames
-    auto load = [&](uint4* dst, int c) {
-        const int f = min(c, nch - 1) * kChunk;
-#pragma unroll
-        for (int k = 0; k < kMaxUnits; k++)
-            if (k * 256 < nunits) dst[k] = *reinterpret_cast<const uint4*>(usrc[k] + f);
-    };
-    // Set (val 1) or clear (val 0) the ones of the thread's units of chunk c
-    // in buffer `boff`.  Only the last chunk can hold padding frames (bin
-    // index 255), so only it tests the index.
-    auto scatter = [&](int boff, const uint4* src, int c, unsigned char val) {
-        const bool check = (c + 1) * kChunk > N;
-#pragma unroll
-        for (int k = 0; k < kMaxUnits; k++) {
-            if (k * 256 >= nunits || !uact[k]) continue;
-            const unsigned words[4] = {src[k].x, src[k].y, src[k].z, src[k].w};
-            unsigned char* base = smem + boff + ubase[k];
-            if (!check) {
-#pragma unroll
-                for (int e = 0; e < 16; e++) {
-                    const unsigned b = (words[e >> 2] >> (8 * (e & 3))) & 0xFFu;
-                    base[__umul24(b, kStride) + (e >> 2) * 32 + (e & 3)] = val;
-                }
-            } else {
-#pragma unroll
-                for (int e = 0; e < 16; e++) {
-                    const unsigned b = (words[e >> 2] >> (8 * (e & 3))) & 0xFFu;
-                    if (b < (unsigned)B) base[__umul24(b, kStride) + (e >> 2) * 32 + (e & 3)] = val;
-                }
-            }
-        }
-    };
-    uint4 w[kMaxUnits], wn[kMaxUnits], wn2[kMaxUnits];
-    load(w, 0);
-    load(wn, 1);
-    lds_barrier();
-    scatter(0, w, 0, 1);
-    lds_barrier();
-
     v16i acc[2][2];
 #pragma unroll
     for (int i = 0; i < 2; i++)
@@ -128,38 +102,215 @@ __global__ __launch_bounds__(256, 2) void k_pairs_mfma(PairInputs in0, int B, in
         for (int j = 0; j < 2; j++)
 #pragma unroll
             for (int v = 0; v < 16; v++) acc[i][j][v] = 0;
-
-    // Chunk c multiplies out of buffer c % 2 while chunk c + 1 is scattered
-    // into the other buffer; after the barrier each thread clears its own
-    // bytes of chunk c (no other thread writes them), so every buffer is
-    // clean again before its next scatter.  The barriers wait for LDS traffic
-    // only: the global loads of chunk c + 2 stay in flight across them.
-    for (int c = 0; c < nch; c++) {
-        const int p = (c & 1) * kBuf, q = kBuf - p;
-        const unsigned char* opA = smem + p;
-        const unsigned char* opB = smem + p + kTile * kStride;
+    if constexpr (PERM != 0) {
+        // Staging unit u = (side, dof slot, frame group of 16): one uint4 of
+        // the dof's bin-index row per chunk, stored at its slot's row of the
+        // side's staging block.  Dof slots past the last dof read dof D-1
+        // (rows no epilogue or dump reads); units past 2 * TD * 8 load the
+        // row dof's first group and store nothing.
+        const int side_bytes = TD * kIdxStride, buf_bytes = 2 * side_bytes;
+        const int nunits = 2 * TD * 8;
+        int ubase[UPT];
+        const uint8_t* usrc[UPT];
 #pragma unroll
-        for (int ks = 0; ks < kChunk / 32; ks++) {
-            v4i a[2], b[2];
-#pragma unroll
-            for (int i = 0; i < 2; i++)
-                a[i] = *reinterpret_cast<const v4i*>(opA + (wr * 64 + i * 32 + (lane & 31)) * kStride + ks * 32 + (lane >> 5) * 16);
-#pragma unroll
-            for (int j = 0; j < 2; j++)
-                b[j] = *reinterpret_cast<const v4i*>(opB + (wc * 64 + j * 32 + (lane & 31)) * kStride + ks * 32 + (lane >> 5) * 16);
-#pragma unroll
-            for (int i = 0; i < 2; i++)
-#pragma unroll
-                for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[i], b[j], acc[i][j], 0, 0, 0);
+        for (int k = 0; k < UPT; k++) {
+            const int u = tid + k * 256;
+            const int side = u / (TD * 8), rem = u % (TD * 8), d = rem / 8, g = rem % 8;
+            const bool act = u < nunits;
+            const int dof = act ? min((side ? c0 : r0) + d, D - 1) : r0;
+            usrc[k] = in.bins + (size_t)dof * in.Np + (act ? g * 16 : 0);
+            ubase[k] = act ? side * side_bytes + d * kIdxStride + g * 16 : -1;
         }
-        if (c + 1 < nch) scatter(q, wn, c + 1, 1);
-        load(wn2, c + 2);
-        lds_barrier();
-        scatter(p, w, c, 0);
+        const int nch = (N + kChunk - 1) / kChunk;
+        auto load = [&](uint4* dst, int c) {
+            const int f = min(c, nch - 1) * kChunk;
 #pragma unroll
+            for (int k = 0; k < UPT; k++) dst[k] = *reinterpret_cast<const uint4*>(usrc[k] + f);
+        };
+        // Only the last chunk holds padding frames (index 255, the only
+        // index with its top bit set): they are staged as kPadIndex.
+        auto pad = [](unsigned x) {
+            const unsigned m = ((x >> 7) & 0x01010101u) * 0xFFu;
+            return (x & ~m) | (kPadIndex * 0x01010101u & m);
+        };
+        auto stage = [&](int buf, const uint4* src, int c) {
+            const bool check = (c + 1) * kChunk > N;
+#pragma unroll
+            for (int k = 0; k < UPT; k++) {
+                if (ubase[k] < 0) continue;
+                uint4 x = src[k];
+                if (check) x = make_uint4(pad(x.x), pad(x.y), pad(x.z), pad(x.w));
+                *reinterpret_cast<uint4*>(smem + buf * buf_bytes + ubase[k]) = x;
+            }
+        };
+        // The lane's operand rows: A rows wr*64 + i*32 + lane%32 of the row
+        // side, B rows wc*64 + j*32 + lane%32 of the column side; row d*B + k
+        // is the one-hot of bin k of dof slot d.  Lanes 32-63 take k slots
+        // 16-31 of each 32-frame step (frame = k slot within the chunk).
+        unsigned thi[2][2], tlo[2][2];
+        int toff[2][2];
+#pragma unroll
+        for (int s2 = 0; s2 < 2; s2++)
+#pragma unroll
+            for (int i = 0; i < 2; i++) {
+                const int row = (s2 ? wc : wr) * 64 + i * 32 + (lane & 31);
+                const bool ok = row < TD * B;
+                onehot_table(ok, row % B, &thi[s2][i], &tlo[s2][i]);
+                toff[s2][i] = s2 * side_bytes + (ok ? row / B : 0) * kIdxStride + (lane >> 5) * 16;
+            }
+        auto onehot = [&](int s2, int i, const uint4& x) {
+            const unsigned words[4] = {x.x, x.y, x.z, x.w};
+            v4i r;
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const unsigned o = __builtin_amdgcn_perm(thi[s2][i], tlo[s2][i], words[e]);
+                r[e] = (int)(PERM == 2 ? o & 0x01010101u : o);
+            }
+            return r;
+        };
+        // Chunk c multiplies out of staging buffer c % 2; chunk c + 1 is
+        // staged into the other one, which every wave finished reading before
+        // the previous barrier.  The barrier waits for LDS traffic only.  The
+        // loop is unrolled by two so that the register sets swap roles
+        // without copies: x holds chunk c + 1 at even c, y at odd c, and each
+        // is reloaded (chunk c + 3) right after it is staged, two chunks of
+        // multiplies before it is needed.
+        uint4 x[UPT], y[UPT];
+        load(y, 0);
+        stage(0, y, 0);
+        load(x, 1);
+        load(y, 2);
+        lds_barrier();
+        // Every step issues its loads, also the one past the last chunk when
+        // nch is odd, so that all paths into the loop head have the same
+        // loads in flight and each wait counts them exactly.
+        auto step = [&](int c, uint4* next) {
+            if (c < nch) {
+                const unsigned char* st = smem + (c & 1) * buf_bytes;
+                // PERM 1 reads all of the chunk's operand words first (4% faster
+                // at 5 bins); with the mask (PERM 2) the compiler's own
+                // interleaving of reads and multiplies measured 1% faster
+                uint4 raw[kChunk / 32][2][2];
+#pragma unroll
+                for (int ks = 0; ks < kChunk / 32; ks++)
+#pragma unroll
+                    for (int s2 = 0; s2 < 2; s2++)
+#pragma unroll
+                        for (int i = 0; i < 2; i++) raw[ks][s2][i] = *reinterpret_cast<const uint4*>(st + toff[s2][i] + ks * 32);
+                if constexpr (PERM == 1) __builtin_amdgcn_sched_barrier(0);  // keep the reads above the multiplies
+#pragma unroll
+                for (int ks = 0; ks < kChunk / 32; ks++) {
+                    v4i a[2], b[2];
+#pragma unroll
+                    for (int i = 0; i < 2; i++) a[i] = onehot(0, i, raw[ks][0][i]);
+#pragma unroll
+                    for (int j = 0; j < 2; j++) b[j] = onehot(1, j, raw[ks][1][j]);
+#pragma unroll
+                    for (int i = 0; i < 2; i++)
+#pragma unroll
+                        for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[i], b[j], acc[i][j], 0, 0, 0);
+                }
+                if (c + 1 < nch) stage((c + 1) & 1, next, c + 1);
+            }
+            load(next, c + 3);
+            lds_barrier();
+        };
+        for (int c = 0; c < nch; c += 2) {
+            step(c, x);
+            step(c + 1, y);
+        }
+    } else {
+        for (int i = tid; i < kOperandBytes / 16; i += 256) reinterpret_cast<uint4*>(smem)[i] = make_uint4(0, 0, 0, 0);
+
+        // Static unit assignment: unit u = (side, dof slot, frame group of 16).
+        // Units of dof slots past the last dof read dof D-1 and fill operand rows
+        // that no epilogue or dump reads; units past the table (u >= nunits, only
+        // in the last partial wave) are masked.
+        const int nunits = 2 * TD * 8;
+        int ubase[kMaxUnits];            // byte offset of the unit's (row block, frame group) origin
+        const uint8_t* usrc[kMaxUnits];  // the dof's bin-index row at the unit's frame group
+        bool uact[kMaxUnits];
+    #pragma unroll
         for (int k = 0; k < kMaxUnits; k++) {
-            w[k] = wn[k];
-            wn[k] = wn2[k];
+            const int u = tid + k * 256;
+            const int side = u / (TD * 8), rem = u % (TD * 8), d = rem / 8, g = rem % 8;
+            uact[k] = u < nunits;
+            const int dof = uact[k] ? min((side ? c0 : r0) + d, D - 1) : r0;
+            usrc[k] = in.bins + (size_t)dof * in.Np + g * 16;
+            ubase[k] = uact[k] ? (side ? kTile * kStride : 0) + d * B * kStride + g * 4 : 0;
+        }
+        const int nch = (N + kChunk - 1) / kChunk;  // chunks holding real frames
+        auto load = [&](uint4* dst, int c) {
+            const int f = min(c, nch - 1) * kChunk;
+    #pragma unroll
+            for (int k = 0; k < kMaxUnits; k++)
+                if (k * 256 < nunits) dst[k] = *reinterpret_cast<const uint4*>(usrc[k] + f);
+        };
+        // Set (val 1) or clear (val 0) the ones of the thread's units of chunk c
+        // in buffer `boff`.  Only the last chunk can hold padding frames (bin
+        // index 255), so only it tests the index.
+        auto scatter = [&](int boff, const uint4* src, int c, unsigned char val) {
+            const bool check = (c + 1) * kChunk > N;
+    #pragma unroll
+            for (int k = 0; k < kMaxUnits; k++) {
+                if (k * 256 >= nunits || !uact[k]) continue;
+                const unsigned words[4] = {src[k].x, src[k].y, src[k].z, src[k].w};
+                unsigned char* base = smem + boff + ubase[k];
+                if (!check) {
+    #pragma unroll
+                    for (int e = 0; e < 16; e++) {
+                        const unsigned b = (words[e >> 2] >> (8 * (e & 3))) & 0xFFu;
+                        base[__umul24(b, kStride) + (e >> 2) * 32 + (e & 3)] = val;
+                    }
+                } else {
+    #pragma unroll
+                    for (int e = 0; e < 16; e++) {
+                        const unsigned b = (words[e >> 2] >> (8 * (e & 3))) & 0xFFu;
+                        if (b < (unsigned)B) base[__umul24(b, kStride) + (e >> 2) * 32 + (e & 3)] = val;
+                    }
+                }
+            }
+        };
+        uint4 w[kMaxUnits], wn[kMaxUnits], wn2[kMaxUnits];
+        load(w, 0);
+        load(wn, 1);
+        lds_barrier();
+        scatter(0, w, 0, 1);
+        lds_barrier();
+
+
+        // Chunk c multiplies out of buffer c % 2 while chunk c + 1 is scattered
+        // into the other buffer; after the barrier each thread clears its own
+        // bytes of chunk c (no other thread writes them), so every buffer is
+        // clean again before its next scatter.  The barriers wait for LDS traffic
+        // only: the global loads of chunk c + 2 stay in flight across them.
+        for (int c = 0; c < nch; c++) {
+            const int p = (c & 1) * kBuf, q = kBuf - p;
+            const unsigned char* opA = smem + p;
+            const unsigned char* opB = smem + p + kTile * kStride;
+    #pragma unroll
+            for (int ks = 0; ks < kChunk / 32; ks++) {
+                v4i a[2], b[2];
+    #pragma unroll
+                for (int i = 0; i < 2; i++)
+                    a[i] = *reinterpret_cast<const v4i*>(opA + (wr * 64 + i * 32 + (lane & 31)) * kStride + ks * 32 + (lane >> 5) * 16);
+    #pragma unroll
+                for (int j = 0; j < 2; j++)
+                    b[j] = *reinterpret_cast<const v4i*>(opB + (wc * 64 + j * 32 + (lane & 31)) * kStride + ks * 32 + (lane >> 5) * 16);
+    #pragma unroll
+                for (int i = 0; i < 2; i++)
+    #pragma unroll
+                    for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[i], b[j], acc[i][j], 0, 0, 0);
+            }
+            if (c + 1 < nch) scatter(q, wn, c + 1, 1);
+            load(wn2, c + 2);
+            lds_barrier();
+            scatter(p, w, c, 0);
+    #pragma unroll
+            for (int k = 0; k < kMaxUnits; k++) {
+                w[k] = wn[k];
+                wn[k] = wn2[k];
+            }
         }
     }
     __syncthreads();
@@ -217,18 +368,49 @@ hipError_t launch_pairs_mfma(const PairInputs& in, int bins, const TileQueue& q,
     const int ntiles = q.ntiles;
     const int TD = mfma_tile_dofs(bins);
     if (TD < 1) return hipErrorInvalidValue;
-    const void* fn = q.descs ? reinterpret_cast<const void*>(&k_pairs_mfma<true>)
-                             : reinterpret_cast<const void*>(&k_pairs_mfma<false>);
-    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, kLds);
+    const int perm = bins <= 8 ? 1 : bins <= kPermMaxBins ? 2 : 0;
+    const int upt = (2 * TD * 8 + 255) / 256;  // 1, 2, 3 or 4; 3 runs as 4
+    const bool batch = q.descs != nullptr;
+    // kernel index: 0 scatter, 1-3 PERM 1 with 1, 2, 4 units per thread, 4 PERM 2 (TD <= 14: 1 unit)
+    const int kid = perm == 0 ? 0 : perm == 2 ? 4 : upt == 1 ? 1 : upt == 2 ? 2 : 3;
+    const void* fns[2][5] = {{reinterpret_cast<const void*>(&k_pairs_mfma<false, 0, 4>),
+                              reinterpret_cast<const void*>(&k_pairs_mfma<false, 1, 1>),
+                              reinterpret_cast<const void*>(&k_pairs_mfma<false, 1, 2>),
+                              reinterpret_cast<const void*>(&k_pairs_mfma<false, 1, 4>),
+                              reinterpret_cast<const void*>(&k_pairs_mfma<false, 2, 1>)},
+                             {reinterpret_cast<const void*>(&k_pairs_mfma<true, 0, 4>),
+                              reinterpret_cast<const void*>(&k_pairs_mfma<true, 1, 1>),
+                              reinterpret_cast<const void*>(&k_pairs_mfma<true, 1, 2>),
+                              reinterpret_cast<const void*>(&k_pairs_mfma<true, 1, 4>),
+                              reinterpret_cast<const void*>(&k_pairs_mfma<true, 2, 1>)}};
+    if (perm == 2 && upt != 1) return hipErrorInvalidValue;
+    const int lds = perm ? kPermLds : kLds;
+    hipError_t e = hipFuncSetAttribute(fns[batch][kid], hipFuncAttributeMaxDynamicSharedMemorySize, lds);
     if (e != hipSuccess) return e;
     const int per_launch = mfma_tiles_per_launch();
     for (int t0 = 0; t0 < ntiles; t0 += per_launch) {
         const int t1 = t0 + per_launch < ntiles ? t0 + per_launch : ntiles;
-        const int chunk = (t1 - t0 + 7) / 8;
-        if (q.descs)
-            k_pairs_mfma<true><<<chunk * 8, 256, kLds, s>>>(in, bins, TD, q, t0, t1, dump_begin, dump_entries, dump_out);
-        else
-            k_pairs_mfma<false><<<chunk * 8, 256, kLds, s>>>(in, bins, TD, q, t0, t1, dump_begin, dump_entries, dump_out);
+        const dim3 grid((t1 - t0 + 7) / 8 * 8);
+#define PARENT_MFMA_LAUNCH(BT, PM, U) \
+    k_pairs_mfma<BT, PM, U><<<grid, 256, lds, s>>>(in, bins, TD, q, t0, t1, dump_begin, dump_entries, dump_out)
+        if (batch) {
+            switch (kid) {
+                case 1: PARENT_MFMA_LAUNCH(true, 1, 1); break;
+                case 2: PARENT_MFMA_LAUNCH(true, 1, 2); break;
+                case 3: PARENT_MFMA_LAUNCH(true, 1, 4); break;
+                case 4: PARENT_MFMA_LAUNCH(true, 2, 1); break;
+                default: PARENT_MFMA_LAUNCH(true, 0, 4); break;
+            }
+        } else {
+            switch (kid) {
+                case 1: PARENT_MFMA_LAUNCH(false, 1, 1); break;
+                case 2: PARENT_MFMA_LAUNCH(false, 1, 2); break;
+                case 3: PARENT_MFMA_LAUNCH(false, 1, 4); break;
+                case 4: PARENT_MFMA_LAUNCH(false, 2, 1); break;
+                default: PARENT_MFMA_LAUNCH(false, 0, 4); break;
+            }
+        }
+#undef PARENT_MFMA_LAUNCH
     }
     return hipGetLastError();
 }

@@ -100,8 +100,8 @@ def test_sweep_against_oracle():
             _check_vs_oracle(nD, vals, b, eng.s1d(), eng.s2d())
 
 
-@pytest.mark.parametrize("bins,method", [([30] * 6, 3), ([10] * 6, 2), ([7, 13, 29, 5, 40, 17], 1),
-                                         ([50] * 6, 0)])
+@pytest.mark.parametrize("bins,method", [([30] * 6, 3), ([10] * 6, 2), ([5] * 6, 2), ([20] * 6, 2),
+                                         ([7, 13, 29, 5, 40, 17], 1), ([50] * 6, 0)])
 def test_batch_of_uneven_trajectories(bins, method):
     """Trajectories of different dof and frame counts in one launch: each
     trajectory's S1D equals the single engine's bit for bit, S2D and MI
