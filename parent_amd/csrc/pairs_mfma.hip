@@ -6,21 +6,28 @@
 // owns a 128 x 128 output tile: TD = floor(128 / B) row dofs x TD column dofs,
 // rows/columns (dof, bin) packed dof by dof, the 128 - TD*B leftover rows zero.
 //
-// The one-hot operands are never compared lane by lane: for each chunk of 128
-// frames every thread scatters the ones of its (dof, 16-frame group) units
-// into the LDS operand tiles (one byte store per dof-frame), the four waves
-// run 4 k-steps of v_mfma_i32_32x32x32_i8 on 2 x 2 blocks each, and the same
-// thread clears its bytes again.  Padding frames carry bin 255 and set
-// nothing.  The accumulators then go to an LDS count tile and each wave
-// reduces whole pairs with the fused entropy epilogue of the atomic kernels.
+// Per chunk of 128 frames the four waves run 4 k-steps of
+// v_mfma_i32_32x32x32_i8 on 2 x 2 blocks each.  The one-hot operands come
+// from one of two builders:
+//   * up to 12 bins (PERM 1, 2): the chunk's bin indices are staged in LDS
+//     (one 16-byte store per dof and 16 frames) and each lane turns the 16
+//     index bytes of its operand row's dof into the row's one-hot bytes with
+//     one v_perm_b32 table lookup per 4 frames; frame f of the chunk is k
+//     slot f;
+//   * 13 to 64 bins (PERM 0): every thread scatters the ones of its (dof,
+//     16-frame group) units into LDS operand tiles (one byte store per
+//     dof-frame) and clears its bytes again after the multiplies; frame
+//     16g + e sits in k slot 32(e/4) + 4g + e%4, so the 8 frame groups of one
+//     dof write 8 consecutive LDS words in each scatter instruction.
+// Padding frames carry bin 255 and set nothing.  The accumulators then go to
+// an LDS count tile and each wave reduces whole pairs with the fused entropy
+// epilogue of the atomic kernels.
 //
 // Lane layouts used (gfx950 32x32x32 int8): A operand lane l = row l % 32,
 // 16 bytes = k slots 16*(l/32) .. +15; B operand lane l = column l % 32, the
 // same k slots; result value v of lane l = row (v/4)*8 + (l/32)*4 + v%4,
 // column l % 32.  Any bijection of k slots onto frames gives the same sums as
-// long as A and B share it, which they do here: frame 16g + e of a chunk sits
-// in k slot 32(e/4) + 4g + e%4, so the 8 frame groups of one dof write 8
-// consecutive LDS words in each scatter instruction (bank-spread stores).
+// long as A and B share it.
 #include "kernels.h"
 
 namespace parent {
