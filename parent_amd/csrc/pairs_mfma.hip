@@ -49,9 +49,12 @@ constexpr int kPermMaxBins = 12;  // the register-built operands (PERM) serve B 
 constexpr int kIdxStride = kChunk + 16;  // staged index row stride in bytes (PERM)
 constexpr unsigned kPadIndex = 12;       // staged value of padding frames: v_perm selector 12 gives 0x00
 // PERM: two staging buffers of both sides' index rows (at most 64 dofs a
-// side), then the count tile in the same memory
+// side), then the count tile's row phases in the same memory: at most
+// max over B <= 12 of ceil(64 / B) * B = 72 rows (B = 9 and 12)
 constexpr int kPermStage = 2 * 2 * 64 * kIdxStride;
-constexpr int kPermLds = kCountBytes > kPermStage ? kCountBytes : kPermStage;
+constexpr int kPermCountRows = 72;
+constexpr int kPermCountBytes = kPermCountRows * kTile * 4;
+constexpr int kPermLds = kPermCountBytes > kPermStage ? kPermCountBytes : kPermStage;
 
 // The 64-bit v_perm_b32 table {hi, lo} whose byte lookup turns a word of 4
 // bin indices into the 4 one-hot bytes (index == k).  v_perm_b32 selector
@@ -84,9 +87,11 @@ __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(
 // adds a mask).  PERM 0: the one-hot bytes are scattered into LDS operand
 // tiles and cleared again (every B <= 64).  UPT (PERM only): staging units
 // per thread, ceil(2 * TD * 8 / 256); every thread issues all UPT loads, so
-// the waits for a staged chunk count the loads in flight exactly.
+// the waits for a staged chunk count the loads in flight exactly.  PERM
+// kernels with UPT < 4 fit 168 VGPRs and 36 KB of LDS: three workgroups per
+// CU (UPT 4, at 2 and 3 bins, would spill at that register budget).
 template <bool BATCH, int PERM, int UPT>
-__global__ __launch_bounds__(256, 2) void k_pairs_mfma(PairInputs in0, int B, int TD, TileQueue q,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PERM && UPT < 4 ? 3 : 2))) void k_pairs_mfma(PairInputs in0, int B, int TD, TileQueue q,
                                                        int t_begin, int t_end, const int* __restrict__ dump_begin,
                                                        const int* __restrict__ dump_entries, int* __restrict__ dump_out) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
@@ -320,41 +325,52 @@ __global__ __launch_bounds__(256, 2) void k_pairs_mfma(PairInputs in0, int B, in
             }
         }
     }
-    __syncthreads();
+    // Counts to LDS, then the epilogue.  The register-built kernels (PERM)
+    // hold the count tile in two row phases, dof slots [0, D0) then [D0, TD)
+    // with D0 = ceil(64 / B), each at most kPermCountRows rows, so that their
+    // LDS is 36 KB and three workgroups share a CU.  PERM 0 writes the whole
+    // 128 x 128 tile in one phase.
+    constexpr int kPhases = PERM ? 2 : 1;
+    const int D0 = PERM ? (64 + B - 1) / B : TD;
+    for (int h = 0; h < kPhases; h++) {
+        const int dlo = h ? D0 : 0, dhi = h ? TD : D0;
+        const int rlo = dlo * B, rhi = h == kPhases - 1 ? kTile : D0 * B;
+        __syncthreads();  // the staging buffers or the previous phase are no longer read
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++)
+#pragma unroll
+                for (int v = 0; v < 16; v++) {
+                    const int row = wr * 64 + i * 32 + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
+                    const int col = wc * 64 + j * 32 + (lane & 31);
+                    if (row >= rlo && row < rhi) cnt[(row - rlo) * kTile + col] = acc[i][j][v];
+                }
+        __syncthreads();
 
-    // counts to LDS: count tile [128][128] int32
-#pragma unroll
-    for (int i = 0; i < 2; i++)
-#pragma unroll
-        for (int j = 0; j < 2; j++)
-#pragma unroll
-            for (int v = 0; v < 16; v++) {
-                const int row = wr * 64 + i * 32 + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
-                const int col = wc * 64 + j * 32 + (lane & 31);
-                cnt[row * kTile + col] = acc[i][j][v];
+        if (dump_out) {
+            for (int e = dump_begin[t]; e < dump_begin[t + 1]; e++) {
+                const int sl = dump_entries[3 * e];
+                const long long off = ((long long)dump_entries[3 * e + 1] << 31) | (long long)dump_entries[3 * e + 2];
+                const int di = sl / TD, dj = sl % TD;
+                if (di < dlo || di >= dhi) continue;
+                const int Bi = in.B.of(dof_type(nD, r0 + di)), Bj = in.B.of(dof_type(nD, c0 + dj));
+                for (int c = tid; c < Bi * Bj; c += 256)
+                    dump_out[off + c] = cnt[((di - dlo) * B + c / Bj) * kTile + dj * B + c % Bj];
             }
-    __syncthreads();
-
-    if (dump_out) {
-        for (int e = dump_begin[t]; e < dump_begin[t + 1]; e++) {
-            const int sl = dump_entries[3 * e];
-            const long long off = ((long long)dump_entries[3 * e + 1] << 31) | (long long)dump_entries[3 * e + 2];
-            const int di = sl / TD, dj = sl % TD;
-            const int Bi = in.B.of(dof_type(nD, r0 + di)), Bj = in.B.of(dof_type(nD, c0 + dj));
-            for (int c = tid; c < Bi * Bj; c += 256) dump_out[off + c] = cnt[(di * B + c / Bj) * kTile + dj * B + c % Bj];
+            continue;
         }
-        return;
-    }
-    for (int p = wid; p < TD * TD; p += 4) {
-        const int di = p / TD, dj = p % TD;
-        const int gi = r0 + di, gj = c0 + dj;
-        if (!(gi < gj && gj < D)) continue;
-        double T = 0.0;
-        // branch-free (the table is 0 at n = 0): the unrolled table loads overlap
+        for (int p = wid; p < (dhi - dlo) * TD; p += 4) {
+            const int di = dlo + p / TD, dj = p % TD;
+            const int gi = r0 + di, gj = c0 + dj;
+            if (!(gi < gj && gj < D)) continue;
+            double T = 0.0;
+            // branch-free (the table is 0 at n = 0): the unrolled table loads overlap
 #pragma unroll 4
-        for (int c = lane; c < B * B; c += 64) T += in.nlogn[cnt[(di * B + c / B) * kTile + dj * B + c % B]];
-        T = wave_total(T);
-        if (lane == 0) in.s2d[par_offset(nD, gi, gj)] = ((in.logN + in.c2[gi]) + in.c2[gj] - 0.5 / N) - T / N;
+            for (int c = lane; c < B * B; c += 64) T += in.nlogn[cnt[((di - dlo) * B + c / B) * kTile + dj * B + c % B]];
+            T = wave_total(T);
+            if (lane == 0) in.s2d[par_offset(nD, gi, gj)] = ((in.logN + in.c2[gi]) + in.c2[gj] - 0.5 / N) - T / N;
+        }
     }
 }
 
@@ -391,6 +407,7 @@ hipError_t launch_pairs_mfma(const PairInputs& in, int bins, const TileQueue& q,
                               reinterpret_cast<const void*>(&k_pairs_mfma<true, 1, 4>),
                               reinterpret_cast<const void*>(&k_pairs_mfma<true, 2, 1>)}};
     if (perm == 2 && upt != 1) return hipErrorInvalidValue;
+    if (perm && (64 + bins - 1) / bins * bins > kPermCountRows) return hipErrorInvalidValue;
     const int lds = perm ? kPermLds : kLds;
     hipError_t e = hipFuncSetAttribute(fns[batch][kid], hipFuncAttributeMaxDynamicSharedMemorySize, lds);
     if (e != hipSuccess) return e;
