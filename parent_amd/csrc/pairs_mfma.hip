@@ -88,10 +88,10 @@ __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(
 // tiles and cleared again (every B <= 64).  UPT (PERM only): staging units
 // per thread, ceil(2 * TD * 8 / 256); every thread issues all UPT loads, so
 // the waits for a staged chunk count the loads in flight exactly.  PERM
-// kernels with UPT < 4 fit 168 VGPRs and 36 KB of LDS: three workgroups per
-// CU (UPT 4, at 2 and 3 bins, would spill at that register budget).
+// kernels with UPT < 4 fit 128 VGPRs and 36 KB of LDS: four workgroups per
+// CU (UPT 4, at 2 and 3 bins, three: it would spill at 128).
 template <bool BATCH, int PERM, int UPT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PERM && UPT < 4 ? 3 : 2))) void k_pairs_mfma(PairInputs in0, int B, int TD, TileQueue q,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PERM ? (UPT < 4 ? 4 : 3) : 2))) void k_pairs_mfma(PairInputs in0, int B, int TD, TileQueue q,
                                                        int t_begin, int t_end, const int* __restrict__ dump_begin,
                                                        const int* __restrict__ dump_entries, int* __restrict__ dump_out) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
@@ -199,24 +199,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PERM && UPT
         auto step = [&](int c, uint4* next) {
             if (c < nch) {
                 const unsigned char* st = smem + (c & 1) * buf_bytes;
-                // PERM 1 reads all of the chunk's operand words first (4% faster
-                // at 5 bins); with the mask (PERM 2) the compiler's own
-                // interleaving of reads and multiplies measured 1% faster
-                uint4 raw[kChunk / 32][2][2];
-#pragma unroll
-                for (int ks = 0; ks < kChunk / 32; ks++)
-#pragma unroll
-                    for (int s2 = 0; s2 < 2; s2++)
-#pragma unroll
-                        for (int i = 0; i < 2; i++) raw[ks][s2][i] = *reinterpret_cast<const uint4*>(st + toff[s2][i] + ks * 32);
-                if constexpr (PERM == 1) __builtin_amdgcn_sched_barrier(0);  // keep the reads above the multiplies
+                // (Issuing all 16 operand reads of the chunk first, pinned by a
+                // scheduling barrier, needs 64 more VGPRs: at three waves per
+                // SIMD it was 4% slower than this at four.)
 #pragma unroll
                 for (int ks = 0; ks < kChunk / 32; ks++) {
                     v4i a[2], b[2];
 #pragma unroll
-                    for (int i = 0; i < 2; i++) a[i] = onehot(0, i, raw[ks][0][i]);
+                    for (int i = 0; i < 2; i++) a[i] = onehot(0, i, *reinterpret_cast<const uint4*>(st + toff[0][i] + ks * 32));
 #pragma unroll
-                    for (int j = 0; j < 2; j++) b[j] = onehot(1, j, raw[ks][1][j]);
+                    for (int j = 0; j < 2; j++) b[j] = onehot(1, j, *reinterpret_cast<const uint4*>(st + toff[1][j] + ks * 32));
 #pragma unroll
                     for (int i = 0; i < 2; i++)
 #pragma unroll
