@@ -38,6 +38,23 @@ __global__ void k_fill_u32(unsigned* p, size_t n, unsigned v) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }
 
+// Visits dof g's values at frames f = f_begin, f_begin + 4, ... < f_end of
+// the frame-major array v (D values a frame), loading KB of them before using
+// any, so that KB loads per thread are in flight (one at a time, each load's
+// latency was exposed).  use(f, x) gets the value widened to double.
+template <int KB, class T, class F>
+__device__ __forceinline__ void for_frames(const T* __restrict__ v, int D, int g, int f_begin, int f_end, F&& use) {
+    int f = f_begin;
+    for (; f + 4 * (KB - 1) < f_end; f += 4 * KB) {
+        T xv[KB];
+#pragma unroll
+        for (int u = 0; u < KB; u++) xv[u] = v[(size_t)(f + 4 * u) * D + g];
+#pragma unroll
+        for (int u = 0; u < KB; u++) use(f + 4 * u, (double)xv[u]);
+    }
+    for (; f < f_end; f += 4) use(f, (double)v[(size_t)f * D + g]);
+}
+
 // Pass 1: min/max of every dof over a slice of frames (exact in the input
 // precision), plus the 100-bin modfit histogram of each dihedral
 // (PARENT.cpp:513-521, 537-545, 568-572).  Block = 64 dofs x 4 frame lanes;
@@ -62,8 +79,8 @@ __global__ __launch_bounds__(256) void k_scan_raw(const T* __restrict__ v, int N
     if (g < D) {
         const bool isd = g >= dih0;
         bool bad = false;
-        for (int f = f0 + fl; f < f1; f += 4) {
-            const double x = (double)v[(size_t)f * D + g];
+        // (min, max and the integer histogram do not depend on the order)
+        for_frames<8>(v, D, g, f0 + fl, f1, [&](int, double x) {
             mn = x < mn ? x : mn;
             mx = x > mx ? x : mx;
             if (isd) {
@@ -71,7 +88,7 @@ __global__ __launch_bounds__(256) void k_scan_raw(const T* __restrict__ v, int N
                 if (k < 0 || k >= kModfitBins) bad = true;
                 else atomicAdd(&hist[dl * kModfitBins + k], 1u);
             }
-        }
+        });
         if (bad) atomicOr(flags, kFlagRange);
     }
     smin[threadIdx.x] = mn;
@@ -178,11 +195,11 @@ __global__ __launch_bounds__(256) void k_scan_dihedral(const T* __restrict__ v, 
     const int f0 = blockIdx.y * fpb, f1 = min(N, f0 + fpb);
     if (g < D) {
         const double mf = modfit[g];
-        for (int f = f0 + fl; f < f1; f += 4) {
-            const double x = modfit_shift((double)v[(size_t)f * D + g], mf, twopi);
+        for_frames<8>(v, D, g, f0 + fl, f1, [&](int, double raw) {
+            const double x = modfit_shift(raw, mf, twopi);
             mn = x < mn ? x : mn;
             mx = x > mx ? x : mx;
-        }
+        });
     }
     smin[threadIdx.x] = mn;
     smax[threadIdx.x] = mx;
@@ -219,7 +236,7 @@ __global__ void k_finalize_dihedral(int D, int nD, double twopi,
 // (PARENT.cpp:654-656 and the 2D fills, 802-804 ...), written transposed to
 // the dof-major rows through an LDS tile of 64 dofs x 256 frames so both the
 // frame-major reads and the dof-major writes are coalesced.
-template <class T>
+template <class T, bool NEED1>
 __global__ __launch_bounds__(256) void k_prebin(const T* __restrict__ v, int N, int Np, int D, int nD,
                                                 const double* __restrict__ vmin,
                                                 const double* __restrict__ vmax,
@@ -229,7 +246,7 @@ __global__ __launch_bounds__(256) void k_prebin(const T* __restrict__ v, int N, 
                                                 int* flags) {
     constexpr int kRow = 260;  // 256 frames + 4: rows start in different LDS banks
     __shared__ uint8_t tile2[64 * kRow];
-    __shared__ uint8_t tile1[64 * kRow];
+    __shared__ uint8_t tile1[NEED1 ? 64 * kRow : 4];
     const int dl = threadIdx.x & 63, fl = threadIdx.x >> 6;
     const int d0 = blockIdx.x * 64, f0 = blockIdx.y * 256;
     const int g = d0 + dl;
@@ -246,35 +263,37 @@ __global__ __launch_bounds__(256) void k_prebin(const T* __restrict__ v, int N, 
         bs1 = (vmax[g] - vmin[g]) / nb1;
         isd = g >= dih0;
         mf = isd ? modfit[g] : 0.0;
-        sep1 = need1 && nb1 != nb2;
+        sep1 = NEED1 && nb1 != nb2;
     }
     bool bad = false;
-    for (int r = fl; r < 256; r += 4) {
-        const int f = f0 + r;
-        uint8_t i2 = kPadBin, i1 = kPadBin;
-        if (g < D && f < N) {
-            double x = (double)v[(size_t)f * D + g];
-            if (isd) x = modfit_shift(x, mf, twopi);
-            int k = (int)((x - mn) / bs2);
-            if (k < 0 || k >= nb2) {
-                bad = true;
-                k = k < 0 ? 0 : nb2 - 1;
-            }
-            i2 = (uint8_t)k;
+    // padding: frames past N, and every frame of slots past the last dof
+    const int fend = g < D ? min(N, f0 + 256) : f0;
+    for (int r = fl; r < 256; r += 4)
+        if (f0 + r >= fend) {
+            tile2[dl * kRow + r] = kPadBin;
+            if constexpr (NEED1) tile1[dl * kRow + r] = kPadBin;
+        }
+    for_frames<16>(v, D, g, f0 + fl, fend, [&](int f, double x) {
+        const int r = f - f0;
+        if (isd) x = modfit_shift(x, mf, twopi);
+        int k = (int)((x - mn) / bs2);
+        if (k < 0 || k >= nb2) {
+            bad = true;
+            k = k < 0 ? 0 : nb2 - 1;
+        }
+        tile2[dl * kRow + r] = (uint8_t)k;
+        if constexpr (NEED1) {
+            int k1 = k;
             if (sep1) {
-                int k1 = (int)((x - mn) / bs1);
+                k1 = (int)((x - mn) / bs1);
                 if (k1 < 0 || k1 >= nb1) {
                     bad = true;
                     k1 = k1 < 0 ? 0 : nb1 - 1;
                 }
-                i1 = (uint8_t)k1;
-            } else {
-                i1 = i2;
             }
+            tile1[dl * kRow + r] = (uint8_t)k1;
         }
-        tile2[dl * kRow + r] = i2;
-        tile1[dl * kRow + r] = i1;
-    }
+    });
     if (bad) atomicOr(flags, kFlagRange);
     __syncthreads();
     for (int w = threadIdx.x; w < 64 * 64; w += 256) {
@@ -283,7 +302,7 @@ __global__ __launch_bounds__(256) void k_prebin(const T* __restrict__ v, int N, 
         if (gg >= D) continue;
         const uint32_t word2 = *reinterpret_cast<const uint32_t*>(&tile2[dof * kRow + 4 * wr]);
         *reinterpret_cast<uint32_t*>(&bins2[(size_t)gg * Np + f0 + 4 * wr]) = word2;
-        if (need1) {
+        if constexpr (NEED1) {
             const uint32_t word1 = *reinterpret_cast<const uint32_t*>(&tile1[dof * kRow + 4 * wr]);
             *reinterpret_cast<uint32_t*>(&bins1[(size_t)gg * Np + f0 + 4 * wr]) = word1;
         }
@@ -440,10 +459,15 @@ hipError_t launch_prepare(const PrepareArgs& a, hipStream_t s) {
         k_finalize_dihedral<<<(nDih + 255) / 256, 256, 0, s>>>(D, nD, twopi, a.emin, a.emax, a.vmin, a.vmax);
     }
     dim3 g5((D + 63) / 64, a.Np / 256);
-    if (a.double_prec)
-        k_prebin<double><<<g5, 256, 0, s>>>((const double*)a.values, N, a.Np, D, nD, a.vmin, a.vmax, a.modfit, twopi, a.B2, a.B1, a.bins1 != nullptr, a.bins2, a.bins1, a.flags);
+    const bool need1 = a.bins1 != nullptr;
+    if (a.double_prec && need1)
+        k_prebin<double, true><<<g5, 256, 0, s>>>((const double*)a.values, N, a.Np, D, nD, a.vmin, a.vmax, a.modfit, twopi, a.B2, a.B1, 1, a.bins2, a.bins1, a.flags);
+    else if (a.double_prec)
+        k_prebin<double, false><<<g5, 256, 0, s>>>((const double*)a.values, N, a.Np, D, nD, a.vmin, a.vmax, a.modfit, twopi, a.B2, a.B1, 0, a.bins2, a.bins1, a.flags);
+    else if (need1)
+        k_prebin<float, true><<<g5, 256, 0, s>>>((const float*)a.values, N, a.Np, D, nD, a.vmin, a.vmax, a.modfit, twopi, a.B2, a.B1, 1, a.bins2, a.bins1, a.flags);
     else
-        k_prebin<float><<<g5, 256, 0, s>>>((const float*)a.values, N, a.Np, D, nD, a.vmin, a.vmax, a.modfit, twopi, a.B2, a.B1, a.bins1 != nullptr, a.bins2, a.bins1, a.flags);
+        k_prebin<float, false><<<g5, 256, 0, s>>>((const float*)a.values, N, a.Np, D, nD, a.vmin, a.vmax, a.modfit, twopi, a.B2, a.B1, 0, a.bins2, a.bins1, a.flags);
     k_dof_stats<<<D, 256, 0, s>>>(a.bins2, a.bins1, N, a.Np, nD, a.B1, a.B2, a.vmin, a.vmax, a.s1d, a.c2, a.jtab);
     k_nlogn<<<(N + 256) / 256, 256, 0, s>>>(N, a.nlogn);
     return hipGetLastError();
