@@ -38,6 +38,27 @@ __global__ void k_fill_u32(unsigned* p, size_t n, unsigned v) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }
 
+// Block order of the frame-major scans (1D grid of 8 * ceil(nbx * nby / 8)
+// workgroups).  Workgroups go to the 8 XCDs round-robin by id and each XCD
+// has its own L2.  A frame row is D values long, not a multiple of 128 bytes,
+// so neighbouring 64-dof blocks of one frame range share the cache lines at
+// their boundary: id b runs logical block L = (b % 8) * ceil(T / 8) + b / 8
+// (T = nbx * nby, dof block fastest), which gives each XCD a contiguous run
+// of logical blocks and one L2 both sides of each boundary.  The grid holds
+// 8 * ceil(T / 8) ids, so L covers [0, 8 ceil(T / 8)) once; ids with L >= T
+// have no block and return false.
+__device__ __forceinline__ bool xcd_block(int nbx, int nby, int* bx, int* by) {
+    const int T = nbx * nby, per = (T + 7) / 8;
+    const int b = (int)blockIdx.x;
+    const int L = (b % 8) * per + b / 8;
+    if (L >= T) return false;
+    *bx = L % nbx;
+    *by = L / nbx;
+    return true;
+}
+
+inline unsigned xcd_grid(int nbx, int nby) { return 8u * (unsigned)((nbx * nby + 7) / 8); }
+
 // Visits dof g's values at frames f = f_begin, f_begin + 4, ... < f_end of
 // the frame-major array v (D values a frame), loading KB of them before using
 // any, so that KB loads per thread are in flight (one at a time, each load's
@@ -61,21 +82,23 @@ __device__ __forceinline__ void for_frames(const T* __restrict__ v, int D, int g
 // one wave reads 64 consecutive values of a frame row.
 template <class T>
 __global__ __launch_bounds__(256) void k_scan_raw(const T* __restrict__ v, int N, int D, int nD,
-                                                  int fpb, double mf_bin,
+                                                  int fpb, double mf_bin, int nbx, int nby,
                                                   unsigned long long* __restrict__ emin,
                                                   unsigned long long* __restrict__ emax,
                                                   unsigned* __restrict__ mfh, int* flags) {
     __shared__ unsigned hist[64 * kModfitBins];
     __shared__ double smin[256], smax[256];
+    int bx, by;
+    if (!xcd_block(nbx, nby, &bx, &by)) return;
     const int dl = threadIdx.x & 63, fl = threadIdx.x >> 6;
-    const int g = blockIdx.x * 64 + dl;
+    const int g = bx * 64 + dl;
     const int dih0 = 2 * nD + 3;
-    const bool blk_dih = (int)(blockIdx.x * 64 + 63) >= dih0;
+    const bool blk_dih = bx * 64 + 63 >= dih0;
     if (blk_dih)
         for (int i = threadIdx.x; i < 64 * kModfitBins; i += 256) hist[i] = 0;
     __syncthreads();
     double mn = 1.0e300, mx = -1.0e300;
-    const int f0 = blockIdx.y * fpb, f1 = min(N, f0 + fpb);
+    const int f0 = by * fpb, f1 = min(N, f0 + fpb);
     if (g < D) {
         const bool isd = g >= dih0;
         bool bad = false;
@@ -107,7 +130,7 @@ __global__ __launch_bounds__(256) void k_scan_raw(const T* __restrict__ v, int N
     }
     if (blk_dih) {
         for (int i = threadIdx.x; i < 64 * kModfitBins; i += 256) {
-            const int gg = blockIdx.x * 64 + i / kModfitBins;
+            const int gg = bx * 64 + i / kModfitBins;
             if (gg >= dih0 && gg < D && hist[i]) atomicAdd(&mfh[(size_t)(gg - dih0) * kModfitBins + i % kModfitBins], hist[i]);
         }
     }
@@ -183,16 +206,18 @@ __global__ void k_finalize_ranges(int D, int nD, double pi, double mf_bin,
 // Pass 3: extrema of the shifted dihedrals (PARENT.cpp:614-623), FP64.
 template <class T>
 __global__ __launch_bounds__(256) void k_scan_dihedral(const T* __restrict__ v, int N, int D, int nD,
-                                                       int fpb, double twopi,
+                                                       int fpb, double twopi, int nbx, int nby,
                                                        const double* __restrict__ modfit,
                                                        unsigned long long* __restrict__ emin,
                                                        unsigned long long* __restrict__ emax) {
     __shared__ double smin[256], smax[256];
+    int bx, by;
+    if (!xcd_block(nbx, nby, &bx, &by)) return;
     const int dl = threadIdx.x & 63, fl = threadIdx.x >> 6;
     const int dih0 = 2 * nD + 3;
-    const int g = dih0 + blockIdx.x * 64 + dl;
+    const int g = dih0 + bx * 64 + dl;
     double mn = 1.0e300, mx = -1.0e300;
-    const int f0 = blockIdx.y * fpb, f1 = min(N, f0 + fpb);
+    const int f0 = by * fpb, f1 = min(N, f0 + fpb);
     if (g < D) {
         const double mf = modfit[g];
         for_frames<8>(v, D, g, f0 + fl, f1, [&](int, double raw) {
@@ -241,14 +266,16 @@ __global__ __launch_bounds__(256) void k_prebin(const T* __restrict__ v, int N, 
                                                 const double* __restrict__ vmin,
                                                 const double* __restrict__ vmax,
                                                 const double* __restrict__ modfit, double twopi,
-                                                Bins3 B2, Bins3 B1, int need1,
+                                                Bins3 B2, Bins3 B1, int nbx, int nby,
                                                 uint8_t* __restrict__ bins2, uint8_t* __restrict__ bins1,
                                                 int* flags) {
     constexpr int kRow = 260;  // 256 frames + 4: rows start in different LDS banks
     __shared__ uint8_t tile2[64 * kRow];
     __shared__ uint8_t tile1[NEED1 ? 64 * kRow : 4];
+    int bx, by;
+    if (!xcd_block(nbx, nby, &bx, &by)) return;
     const int dl = threadIdx.x & 63, fl = threadIdx.x >> 6;
-    const int d0 = blockIdx.x * 64, f0 = blockIdx.y * 256;
+    const int d0 = bx * 64, f0 = by * 256;
     const int g = d0 + dl;
     const int dih0 = 2 * nD + 3;
     double mn = 0, bs2 = 1, bs1 = 1, mf = 0;
@@ -444,30 +471,33 @@ hipError_t launch_prepare(const PrepareArgs& a, hipStream_t s) {
     if (nDih > 0) k_fill_u32<<<64, 256, 0, s>>>(a.mfhist, (size_t)nDih * kModfitBins, 0u);
     k_fill_u32<<<1, 32, 0, s>>>((unsigned*)a.flags, 1, 0u);
     const int fpb = 2048;
-    dim3 g1((D + 63) / 64, (N + fpb - 1) / fpb);
+    const int nbx1 = (D + 63) / 64, nby1 = (N + fpb - 1) / fpb;
+    const unsigned g1 = xcd_grid(nbx1, nby1);
     if (a.double_prec)
-        k_scan_raw<double><<<g1, 256, 0, s>>>((const double*)a.values, N, D, nD, fpb, mf_bin, a.emin, a.emax, a.mfhist, a.flags);
+        k_scan_raw<double><<<g1, 256, 0, s>>>((const double*)a.values, N, D, nD, fpb, mf_bin, nbx1, nby1, a.emin, a.emax, a.mfhist, a.flags);
     else
-        k_scan_raw<float><<<g1, 256, 0, s>>>((const float*)a.values, N, D, nD, fpb, mf_bin, a.emin, a.emax, a.mfhist, a.flags);
+        k_scan_raw<float><<<g1, 256, 0, s>>>((const float*)a.values, N, D, nD, fpb, mf_bin, nbx1, nby1, a.emin, a.emax, a.mfhist, a.flags);
     k_finalize_ranges<<<(D + 255) / 256, 256, 0, s>>>(D, nD, pi, mf_bin, a.emin, a.emax, a.mfhist, a.vmin, a.vmax, a.modfit, a.flags);
     if (nDih > 0) {
-        dim3 g3((nDih + 63) / 64, (N + fpb - 1) / fpb);
+        const int nbx3 = (nDih + 63) / 64, nby3 = (N + fpb - 1) / fpb;
+        const unsigned g3 = xcd_grid(nbx3, nby3);
         if (a.double_prec)
-            k_scan_dihedral<double><<<g3, 256, 0, s>>>((const double*)a.values, N, D, nD, fpb, twopi, a.modfit, a.emin, a.emax);
+            k_scan_dihedral<double><<<g3, 256, 0, s>>>((const double*)a.values, N, D, nD, fpb, twopi, nbx3, nby3, a.modfit, a.emin, a.emax);
         else
-            k_scan_dihedral<float><<<g3, 256, 0, s>>>((const float*)a.values, N, D, nD, fpb, twopi, a.modfit, a.emin, a.emax);
+            k_scan_dihedral<float><<<g3, 256, 0, s>>>((const float*)a.values, N, D, nD, fpb, twopi, nbx3, nby3, a.modfit, a.emin, a.emax);
         k_finalize_dihedral<<<(nDih + 255) / 256, 256, 0, s>>>(D, nD, twopi, a.emin, a.emax, a.vmin, a.vmax);
     }
-    dim3 g5((D + 63) / 64, a.Np / 256);
+    const int nbx5 = (D + 63) / 64, nby5 = a.Np / 256;
+    const unsigned g5 = xcd_grid(nbx5, nby5);
     const bool need1 = a.bins1 != nullptr;
     if (a.double_prec && need1)
-        k_prebin<double, true><<<g5, 256, 0, s>>>((const double*)a.values, N, a.Np, D, nD, a.vmin, a.vmax, a.modfit, twopi, a.B2, a.B1, 1, a.bins2, a.bins1, a.flags);
+        k_prebin<double, true><<<g5, 256, 0, s>>>((const double*)a.values, N, a.Np, D, nD, a.vmin, a.vmax, a.modfit, twopi, a.B2, a.B1, nbx5, nby5, a.bins2, a.bins1, a.flags);
     else if (a.double_prec)
-        k_prebin<double, false><<<g5, 256, 0, s>>>((const double*)a.values, N, a.Np, D, nD, a.vmin, a.vmax, a.modfit, twopi, a.B2, a.B1, 0, a.bins2, a.bins1, a.flags);
+        k_prebin<double, false><<<g5, 256, 0, s>>>((const double*)a.values, N, a.Np, D, nD, a.vmin, a.vmax, a.modfit, twopi, a.B2, a.B1, nbx5, nby5, a.bins2, a.bins1, a.flags);
     else if (need1)
-        k_prebin<float, true><<<g5, 256, 0, s>>>((const float*)a.values, N, a.Np, D, nD, a.vmin, a.vmax, a.modfit, twopi, a.B2, a.B1, 1, a.bins2, a.bins1, a.flags);
+        k_prebin<float, true><<<g5, 256, 0, s>>>((const float*)a.values, N, a.Np, D, nD, a.vmin, a.vmax, a.modfit, twopi, a.B2, a.B1, nbx5, nby5, a.bins2, a.bins1, a.flags);
     else
-        k_prebin<float, false><<<g5, 256, 0, s>>>((const float*)a.values, N, a.Np, D, nD, a.vmin, a.vmax, a.modfit, twopi, a.B2, a.B1, 0, a.bins2, a.bins1, a.flags);
+        k_prebin<float, false><<<g5, 256, 0, s>>>((const float*)a.values, N, a.Np, D, nD, a.vmin, a.vmax, a.modfit, twopi, a.B2, a.B1, nbx5, nby5, a.bins2, a.bins1, a.flags);
     k_dof_stats<<<D, 256, 0, s>>>(a.bins2, a.bins1, N, a.Np, nD, a.B1, a.B2, a.vmin, a.vmax, a.s1d, a.c2, a.jtab);
     k_nlogn<<<(N + 256) / 256, 256, 0, s>>>(N, a.nlogn);
     return hipGetLastError();
