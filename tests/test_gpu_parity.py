@@ -87,6 +87,36 @@ def test_prepare_bit_exact(engine, atoms, frames, seed, bins):
     assert np.max(np.abs(s1 - s1o)) <= 1e-12
 
 
+def test_fp64_trajectory_vs_oracle(engine):
+    """An FP64 trajectory (values no FP32 file could hold) through the double
+    instantiations of the scans and the pre-bin pass, with 1D bin counts that
+    differ from the 2D ones for bonds and angles (a second index set) and
+    equal for dihedrals: ranges and every index bit-exact, S1D and S2D within
+    tolerance of the FP64 oracle."""
+    pa = _pa()
+    atoms, frames, bins = 30, 5001, [50, 40, 30, 20, 25, 30]
+    print(f"fp64 atoms={atoms} frames={frames} bins={bins}", flush=True)
+    vals = pa.synth_values(atoms, frames, 12).astype(np.float64)
+    vals += np.linspace(0, 1e-7, vals.shape[1])
+    nD = atoms - 3
+    engine.set_trajectory(vals)
+    engine.prepare(bins)
+    s1o, s2o, mno, mxo, mfo, shifted = O.compute(nD, frames, vals.T.copy(), bins)
+    mn, mx, mf = engine.ranges()
+    assert np.array_equal(mn.view(np.uint64), mno.view(np.uint64))
+    assert np.array_equal(mx.view(np.uint64), mxo.view(np.uint64))
+    assert np.array_equal(mf.view(np.uint64), mfo.view(np.uint64))
+    gb = engine.bins()
+    for g in range(3 * nD + 3):
+        t = O.type_of(nD, g)
+        idx, bad = O.bin_index(shifted[g], mno[g], mxo[g], bins[3 + t])
+        assert bad == 0
+        assert np.array_equal(gb[g].astype(np.int32), idx), f"2D bin indices differ for dof {g}"
+    assert np.max(np.abs(engine.s1d() - s1o)) <= 1e-12
+    engine.pairs(pa.METHOD_AUTO)
+    assert np.max(np.abs(engine.s2d() - s2o)) <= S_ABS
+
+
 @pytest.mark.parametrize("method", [0, 1])
 @pytest.mark.parametrize("atoms,frames,seed,bins", CASES[:4])
 def test_counts_bit_exact(engine, method, atoms, frames, seed, bins):
