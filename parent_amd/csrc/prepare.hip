@@ -138,16 +138,27 @@ __global__ __launch_bounds__(256) void k_scan_raw(const T* __restrict__ v, int N
 
 // Pass 2: range rules of bonds and angles, and the modfit shift of each
 // dihedral from its 100-bin histogram (PARENT.cpp:522-533, 546-562, 573-609).
-__global__ void k_finalize_ranges(int D, int nD, double pi, double mf_bin,
+// One thread per dof, kFinalizeBlock dofs a block.
+constexpr int kFinalizeBlock = 64;
+__global__ __launch_bounds__(kFinalizeBlock) void k_finalize_ranges(int D, int nD, double pi, double mf_bin,
                                   const unsigned long long* __restrict__ emin,
                                   const unsigned long long* __restrict__ emax,
                                   const unsigned* __restrict__ mfh, double* vmin, double* vmax,
                                   double* modfit, int* flags) {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    // The block's dihedral histograms are staged in LDS with coalesced loads
+    // first: the serial scans below then read LDS, not global memory.
+    __shared__ unsigned sh[kFinalizeBlock * kModfitBins];
+    const int g0 = blockIdx.x * kFinalizeBlock, dih0 = 2 * nD + 3;
+    for (int i = threadIdx.x; i < kFinalizeBlock * kModfitBins; i += kFinalizeBlock) {
+        const int gg = g0 + i / kModfitBins;
+        if (gg >= dih0 && gg < D) sh[i] = mfh[(size_t)(gg - dih0) * kModfitBins + i % kModfitBins];
+    }
+    __syncthreads();
+    const int g = g0 + threadIdx.x;
     if (g >= D) return;
     const int t = dof_type(nD, g);
     if (t == 2) {
-        const unsigned* histo = mfh + (size_t)(g - (2 * nD + 3)) * kModfitBins;
+        const unsigned* histo = sh + threadIdx.x * kModfitBins;
         bool zeroExists = false;
         for (int k = 0; k < kModfitBins; k++) zeroExists = zeroExists || (histo[k] == 0);
         long long longest;
@@ -477,7 +488,7 @@ hipError_t launch_prepare(const PrepareArgs& a, hipStream_t s) {
         k_scan_raw<double><<<g1, 256, 0, s>>>((const double*)a.values, N, D, nD, fpb, mf_bin, nbx1, nby1, a.emin, a.emax, a.mfhist, a.flags);
     else
         k_scan_raw<float><<<g1, 256, 0, s>>>((const float*)a.values, N, D, nD, fpb, mf_bin, nbx1, nby1, a.emin, a.emax, a.mfhist, a.flags);
-    k_finalize_ranges<<<(D + 255) / 256, 256, 0, s>>>(D, nD, pi, mf_bin, a.emin, a.emax, a.mfhist, a.vmin, a.vmax, a.modfit, a.flags);
+    k_finalize_ranges<<<(D + kFinalizeBlock - 1) / kFinalizeBlock, kFinalizeBlock, 0, s>>>(D, nD, pi, mf_bin, a.emin, a.emax, a.mfhist, a.vmin, a.vmax, a.modfit, a.flags);
     if (nDih > 0) {
         const int nbx3 = (nDih + 63) / 64, nby3 = (N + fpb - 1) / fpb;
         const unsigned g3 = xcd_grid(nbx3, nby3);
