@@ -59,7 +59,9 @@ LDS_PEAK_FALLBACK = {"conflict_free": 9.5e12, "random_2500": 5.2e12}  # lds_peak
 # by the elementwise-kernel calibration (FETCH x 2.00, WRITE x 1.00):
 # profiles/pairs_multi_protein_1500x100k_b50.md (k_pairs_multi<1024, 1275, 2, 1>, the autotuner's
 # choice at 50 bins: 63.62 GB read + 0.166 GB written).
-PROFILED_TRAFFIC = {("protein_1500x100k_b50", 1): 63.62e9 + 0.166e9}
+# Keyed by (workload, method, variant): the figure is used only when the step ran that same kernel.
+PROFILED_TRAFFIC = {("protein_1500x100k_b50", 1, 53): (63.62e9 + 0.166e9,
+                                                      "profiles/pairs_multi_protein_1500x100k_b50.md")}
 METRIC = "pair-frame updates/s, all-pairs MI (4,491 dofs,100k fr,50 bins): tuned vs plain"
 
 
@@ -80,69 +82,104 @@ def lds_peak():
         return dict(LDS_PEAK_FALLBACK, source="recorded lds_peak.x figures")
 
 
-def _time_cpu(exe, atoms_sample, frames, bins, nthreads):
+def cpu_host():
+    """The measuring host: lscpu model, sockets, cores, the affinity mask and
+    the OpenMP thread count the CPU legs use (the process's CPU share: every
+    CPU of its affinity mask, capped by OMP_NUM_THREADS when the host sets
+    it for this process)."""
+    info = {"nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0))}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for ln in out.splitlines():
+            k, _, v = ln.partition(":")
+            k, v = k.strip(), v.strip()
+            if k in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core", "CPU(s)"):
+                info[k.lower().replace("(s)", "s").replace(" ", "_")] = v
+    except Exception as ex:  # noqa: BLE001
+        info["lscpu_error"] = str(ex)
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    threads = info["affinity_cpus"]
+    if env.isdigit() and int(env) > 0:
+        threads = min(threads, int(env))
+        info["omp_num_threads_env"] = int(env)
+    info["threads_used"] = threads
+    return info
+
+
+def _native_oracle(td):
+    """The oracle restatement built -O3 -march=native for this host (the
+    image has g++), or None."""
+    exe = os.path.join(td, "oracle_parent_native.x")
+    src = [os.path.join(ROOT, "oracle", "oracle_main.cpp"), os.path.join(ROOT, "oracle", "oracle.cpp")]
+    try:
+        r = subprocess.run(["g++", "-O3", "-march=native", "-fopenmp", "-std=c++17", "-o", exe] + src,
+                           capture_output=True, text=True, timeout=300)
+        return exe if r.returncode == 0 else None
+    except Exception:  # noqa: BLE001
+        return None
+
+
+def _time_cpu(exe, atoms_sample, frames, bins, nthreads, td):
     """`exe` (a PARENT.x-compatible program) on a synthetic chain of
     `atoms_sample` atoms: its 'Calculation time' line (PARENT.cpp:498-1489)
     and the pair-frame updates it covers, or None."""
     import parent_amd
-    with tempfile.TemporaryDirectory() as td:
-        bat = os.path.join(td, "sample.bat")
+    bat = os.path.join(td, f"sample_{atoms_sample}x{frames}.bat")
+    if not os.path.exists(bat):
         parent_amd.synth_write_bat(bat, atoms_sample, frames, 12345)
-        env = dict(os.environ, OMP_NUM_THREADS=str(nthreads))
-        env["LD_LIBRARY_PATH"] = "/opt/conda/lib:" + env.get("LD_LIBRARY_PATH", "")
-        try:
-            r = subprocess.run([exe, bat, os.path.join(td, "o.par")] + [str(bins)] * 6,
-                               capture_output=True, text=True, timeout=600, env=env)
-        except subprocess.TimeoutExpired:
-            return None
-        calc = [ln for ln in r.stdout.splitlines() if ln.startswith("Calculation time:")]
-        if r.returncode != 0 or not calc:
-            log(f"cpu run of {os.path.basename(exe)} failed: rc={r.returncode} {r.stderr[-300:]}")
-            return None
-        D = 3 * atoms_sample - 6
-        return float(calc[-1].split(":")[1]), D * (D - 1) / 2 * frames
-
-
-def cpu_baseline(atoms_sample, frames, bins, nthreads):
-    """Reference PARENT.x (built from its sources, -O3) timed on the host on a
-    bounded sample: a chain of `atoms_sample` atoms with the same frame count
-    and distributions.  Falls back to the oracle restatement."""
-    ref = os.path.join(ROOT, "oracle", "_ref", "PARENT_O3.x")
-    port = os.path.join(ROOT, "oracle", "_build", "oracle_parent.x")
-    for kind, exe in (("reference", ref), ("port", port)):
-        if not os.path.exists(exe):
-            continue
-        t = _time_cpu(exe, atoms_sample, frames, bins, nthreads)
-        if t is None:
-            continue
-        secs, upd = t
-        D = 3 * atoms_sample - 6
-        return {"value": upd / secs, "unit": "pair-frame updates/s", "cores": nthreads, "kind": kind,
-                "sample": f"{atoms_sample}-atom chain ({D} dofs, {D * (D - 1) // 2} pairs) x {frames} "
-                          f"frames, {bins} bins, same distributions; 'Calculation time' span "
-                          f"(PARENT.cpp:498-1489) = {secs:.2f} s; build: "
-                          + ("-O3 -march=x86-64-v3 -fopenmp" if kind == "reference" else "oracle restatement")}
-    return None
-
-
-def cpu_shipped_flags(atoms_sample, frames, bins, nthreads):
-    """The reference PARENT.x built with its shipped flags (-fopenmp -Wall,
-    i.e. -O0), on a smaller bounded sample: the slower of the two reference
-    builds, reported beside the -O3 baseline."""
-    exe = os.path.join(ROOT, "oracle", "_ref", "PARENT.x")
-    if atoms_sample <= 0 or not os.path.exists(exe):
+    env = dict(os.environ, OMP_NUM_THREADS=str(nthreads))
+    try:
+        r = subprocess.run([exe, bat, os.path.join(td, "o.par")] + [str(bins)] * 6,
+                           capture_output=True, text=True, timeout=600, env=env)
+    except subprocess.TimeoutExpired:
         return None
-    t = _time_cpu(exe, atoms_sample, frames, bins, nthreads)
-    if t is None:
+    calc = [ln for ln in r.stdout.splitlines() if ln.startswith("Calculation time:")]
+    if r.returncode != 0 or not calc:
+        log(f"cpu run of {os.path.basename(exe)} failed: rc={r.returncode} {r.stderr[-300:]}")
         return None
-    secs, upd = t
     D = 3 * atoms_sample - 6
-    return {"value": upd / secs, "unit": "pair-frame updates/s", "cores": nthreads, "kind": "reference",
-            "sample": f"{atoms_sample}-atom chain ({D} dofs) x {frames} frames, {bins} bins; 'Calculation "
-                      f"time' = {secs:.2f} s; build: the reference's shipped flags (-fopenmp -Wall, -O0)"}
+    return float(calc[-1].split(":")[1]), D * (D - 1) / 2 * frames
 
 
-def roofline_of(pa, method, bins, updates, kernel_ms, workload, D, P, frames):
+def cpu_baselines(atoms_sample, shipped_atoms, frames, bins):
+    """The CPU path timed on this host: the oracle's restatement of PARENT.x
+    (oracle_parent.x: the reference's per-row OpenMP schedule and FP64
+    on-the-fly binning, PARENT.cpp:774-1479) built (1) -O3 -march=native for
+    this host and (2) with the reference's shipped flags (-fopenmp -Wall, no
+    -O: Makefile:9,50), each on a bounded sample of the workload.  Returns
+    (cpu_baseline, cpu_shipped_flags); either may be None."""
+    host = cpu_host()
+    nthreads = host["threads_used"]
+    log(f"cpu host: {host}")
+    out = []
+    with tempfile.TemporaryDirectory() as td:
+        native = _native_oracle(td)
+        o3 = os.path.join(ROOT, "oracle", "_build", "oracle_parent_O3.x")
+        shipped = os.path.join(ROOT, "oracle", "_build", "oracle_parent_shipped.x")
+        legs = ((native or o3, atoms_sample,
+                 "-O3 -march=native -fopenmp (built on this host)" if native else "-O3 -march=x86-64-v3 -fopenmp"),
+                (shipped, shipped_atoms, "the reference's shipped flags: -fopenmp -Wall, no -O (i.e. -O0)"))
+        for exe, atoms, flags in legs:
+            if atoms <= 0 or not exe or not os.path.exists(exe):
+                out.append(None)
+                continue
+            t = _time_cpu(exe, atoms, frames, bins, nthreads, td)
+            if t is None:
+                out.append(None)
+                continue
+            secs, upd = t
+            D = 3 * atoms - 6
+            log(f"cpu {flags}: {secs:.2f} s on {D} dofs x {frames} frames -> {upd / secs:.3e} updates/s")
+            out.append({"value": upd / secs, "unit": "pair-frame updates/s", "cores": nthreads, "kind": "port",
+                        "sample": f"{atoms}-atom chain ({D} dofs, {D * (D - 1) // 2} pairs) x {frames} frames, "
+                                  f"{bins} bins, same distributions; 'Calculation time' span (PARENT.cpp:498-1489) "
+                                  f"= {secs:.2f} s; oracle restatement of PARENT.x (reference OpenMP schedule) "
+                                  f"built {flags}; {nthreads} OpenMP threads",
+                        "host": host})
+    return out[0], out[1]
+
+
+def roofline_of(pa, method, variant, bins, updates, kernel_ms, workload, D, P, frames):
     """Roofline of the dominant kernel (the pair kernel).  On the LDS-atomic
     paths one pair-frame update is one ds_add_u32, so the bound is the LDS
     atomic rate (measured: no published figure exists); the one-hot int8 MFMA
@@ -160,9 +197,10 @@ def roofline_of(pa, method, bins, updates, kernel_ms, workload, D, P, frames):
              "frac_of_random_scatter_peak": rate / peak["random_2500"],
              "peak_source": peak["source"] + "; conflict-free ds_add_u32, 16 lanes/clk/CU",
              "algorithmic_work_per_launch": f"{updates:.4e} pair-frame updates (1 LDS atomic each)"}
-    traffic = PROFILED_TRAFFIC.get((workload, method))
+    traffic, source = PROFILED_TRAFFIC.get((workload, method, variant), (None, None))
     r["traffic"] = traffic
     if traffic is not None:
+        r["traffic_source"] = f"{source} (method {method} variant {variant}, the kernel this step ran)"
         r["hbm"] = {"achieved": traffic / (kernel_ms / 1e3) / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                     "frac": traffic / (kernel_ms / 1e3) / HBM_PEAK,
                     "algorithmic_bytes": D * (-(-frames // 4096) * 4096) + P * 8}
@@ -255,9 +293,9 @@ def run_single(args, pa, nthreads):
         plain = {"value": updates / (pms / 1e3), "ms_per_step": pms,
                  "speedup": (updates / (ms_step / 1e3)) / (updates / (pms / 1e3))}
 
-    roofline = roofline_of(pa, st["method"], bins, updates, pair_avg, args.workload, D, P, frames)
-    cpu = None if args.no_cpu else cpu_baseline(args.cpu_sample_atoms, frames, bins, nthreads)
-    shipped = None if args.no_cpu else cpu_shipped_flags(args.cpu_shipped_atoms, frames, bins, nthreads)
+    roofline = roofline_of(pa, st["method"], st["kernel_variant"], bins, updates, pair_avg, args.workload, D, P, frames)
+    cpu, shipped = (None, None) if args.no_cpu else cpu_baselines(args.cpu_sample_atoms, args.cpu_shipped_atoms,
+                                                                   frames, bins)
     extra = {"phases": {"prepare_ms": sum(prep_ms) / len(prep_ms), "pairs_ms": pair_avg,
                         "pairs_kernel_updates_per_s": kernel_rate, "first_step_ms": first_ms},
              "plain": plain, "mist": mist, "roofline": roofline, "cpu_baseline": cpu,
@@ -307,7 +345,7 @@ def run_sweep(args, pa, nthreads):
         log(f"bins {b}: method {entry['method']} variant {entry['variant']} "
             + (f"MIST total {entry['mist_total_entropy']:.6f}" if not args.no_mist else ""))
     eng.close()
-    cpu = None if args.no_cpu else cpu_baseline(args.cpu_sample_atoms, frames, 50, nthreads)
+    cpu = None if args.no_cpu else cpu_baselines(args.cpu_sample_atoms, 0, frames, 50)[0]
     if cpu:
         cpu["sample"] += "; 50 bins stands for the sweep (CPU cost per update is flat in B)"
     emit(args, updates / (ms_step / 1e3), ms_step,
@@ -375,8 +413,8 @@ def run_batch(args, pa, nthreads):
             _, _, _, tot = b.mist(k)
             mist.append(float(tot[-1]))
     b.close()
-    roofline = roofline_of(pa, st["method"], bins, updates, pair_avg, "batch64_b30", 0, 0, 0)
-    cpu = None if args.no_cpu else cpu_baseline(args.cpu_sample_atoms, 100_000, bins, nthreads)
+    roofline = roofline_of(pa, st["method"], st["kernel_variant"], bins, updates, pair_avg, "batch64_b30", 0, 0, 0)
+    cpu = None if args.no_cpu else cpu_baselines(args.cpu_sample_atoms, 0, 100_000, bins)[0]
     emit(args, updates / (ms_step / 1e3), ms_step,
          {"workload": "batch64_b30", "trajectories": len(shapes), "bins": bins, "updates": updates,
           "atoms_frames": shapes, "method": st["method"], "variant": st["kernel_variant"]},
@@ -442,7 +480,7 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=1, help="timed steps of the plain kernel (0: skip)")
     ap.add_argument("--cpu-sample-atoms", type=int, default=500)
     ap.add_argument("--cpu-shipped-atoms", type=int, default=200,
-                    help="sample for the reference built with its shipped flags (0: skip)")
+                    help="sample for the restatement built with the reference's shipped flags (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mist", action="store_true")
     ap.add_argument("--tune-report", action="store_true", help="print the autotuner's table")
@@ -453,7 +491,7 @@ def main():
               "the multi-GPU sharding is a design note, not a runnable path.", file=sys.stderr)
         sys.exit(2)
     import parent_amd as pa
-    nthreads = min(16, len(os.sched_getaffinity(0)))
+    nthreads = cpu_host()["threads_used"]  # synthetic-data generation threads
     if args.workload == "sweep_1500x100k":
         run_sweep(args, pa, nthreads)
     elif args.workload == "batch64_b30":

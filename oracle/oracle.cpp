@@ -186,7 +186,8 @@ int oracle_bin_index(int N, const double* x, double vmin, double vmax, int bins,
 // PARENT.cpp:650-671, 694-715, 738-757.
 double oracle_entropy1d(int type, int numFrames, const double* x, double vmin, double vmax,
                         int bins) {
-    std::vector<long long> histo(bins, 0);
+    std::vector<long long> hv(bins, 0);
+    long long* histo = hv.data();  // plain pointer: unoptimised builds index it like the reference's array
     double binsize = (vmax - vmin) / (bins);
     for (int i = 0; i < numFrames; i++) {
         int k = int((x[i] - vmin) / binsize);
@@ -231,23 +232,44 @@ double oracle_entropy1d(int type, int numFrames, const double* x, double vmin, d
 // The six 2D epilogues of PARENT.cpp (rank==i variants, the ones a single
 // rank executes): BB 805-822, BA 838-855, BD 870-885, AA 1158-1175,
 // AD 1190-1205, DD 1400-1413.
-double oracle_entropy2d(int t1, const double* x1, double min1, double max1, int bins1, int t2,
-                        const double* x2, double min2, double max2, int bins2, int numFrames,
-                        int64_t* counts) {
-    std::vector<long long> histo((size_t)bins1 * bins2, 0);
-    double binsize1 = (max1 - min1) / bins1;
-    double binsize2 = (max2 - min2) / bins2;
+}  // extern "C"
+
+namespace {
+
+// The 2D entropy of the pair (row dof a, column dof b) of the dof-major
+// value rows `rows` with ranges mins/maxs, into a caller's histogram buffer
+// of bins1 rows of `stride` >= bins2 long longs, zeroed here over
+// [bins1][bins2] as the reference zeroes its per-thread long long
+// histo[B][biggest] before each pair (PARENT.cpp:790-797).  The fill indexes
+// rows[a][i] and mins[a] like the reference's chunk[j][iOmp] and min[j]
+// (PARENT.cpp:803), so that an unoptimised build (the reference's shipped
+// flags) runs the same memory accesses as the reference's.  CHECK: a value
+// outside its histogram gives NaN (the reference writes out of bounds);
+// without it the caller has checked every index beforehand.
+template <bool CHECK>
+double entropy2d_core(int t1, const double* const* rows, const double* mins, const double* maxs, int a,
+                      int bins1, int t2, int b, int bins2, int numFrames, long long* h, int stride,
+                      int64_t* counts) {
+    for (int k = 0; k < bins1; k++)
+        for (int l = 0; l < bins2; l++) h[(size_t)k * stride + l] = 0;
+    const double min1 = mins[a], min2 = mins[b];
+    double binsize1 = (maxs[a] - mins[a]) / bins1;
+    double binsize2 = (maxs[b] - mins[b]) / bins2;
     for (int i = 0; i < numFrames; i++) {
-        int k = int((x1[i] - min1) / binsize1), l = int((x2[i] - min2) / binsize2);
-        if (k < 0 || k >= bins1 || l < 0 || l >= bins2) return NAN;  // reference: out-of-bounds write
-        histo[(size_t)k * bins2 + l] += 1;
+        if (CHECK) {
+            int k = int((rows[a][i] - mins[a]) / binsize1), l = int((rows[b][i] - mins[b]) / binsize2);
+            if (k < 0 || k >= bins1 || l < 0 || l >= bins2) return NAN;  // reference: out-of-bounds write
+            h[(size_t)k * stride + l] += 1;
+        } else {
+            h[(size_t)int((rows[a][i] - mins[a]) / binsize1) * stride + int((rows[b][i] - mins[b]) / binsize2)] += 1;
+        }
     }
     if (counts)
-        for (size_t c = 0; c < histo.size(); c++) counts[c] = histo[c];
+        for (int k = 0; k < bins1; k++)
+            for (int l = 0; l < bins2; l++) counts[(size_t)k * bins2 + l] = h[(size_t)k * stride + l];
     int occupbins = 0;
     double plnpsum = 0, sum1, probDens;
-    const long long* h = histo.data();
-#define H(k, l) ((double)h[(size_t)(k) * bins2 + (l)])
+#define H(k, l) ((double)h[(size_t)(k) * stride + (l)])
     if (t1 == ORACLE_B && t2 == ORACLE_B) {
         double blen1 = min1 + binsize1 / 2.0, blen2;
         for (int k = 0; k < bins1; k++) {
@@ -341,6 +363,20 @@ double oracle_entropy2d(int t1, const double* x1, double min1, double max1, int 
     return -plnpsum + (occupbins - 1.0) / (2.0 * numFrames);
 }
 
+}  // namespace
+
+extern "C" {
+
+double oracle_entropy2d(int t1, const double* x1, double min1, double max1, int bins1, int t2,
+                        const double* x2, double min2, double max2, int bins2, int numFrames,
+                        int64_t* counts) {
+    std::vector<long long> histo((size_t)bins1 * bins2);
+    const double* rows[2] = {x1, x2};
+    const double mins[2] = {min1, min2}, maxs[2] = {max1, max2};
+    return entropy2d_core<true>(t1, rows, mins, maxs, 0, bins1, t2, 1, bins2, numFrames, histo.data(), bins2,
+                                counts);
+}
+
 // Offsets of the six 2D blocks in the .par body (io_binary.cpp:606-631) and
 // the reverse-order half-matrix index (util.cpp:52-57).
 int64_t oracle_par_pos(int nD, int gi, int gj) {
@@ -389,6 +425,69 @@ int oracle_compute(int nD, int N, double* values, const int bins[6], double* s1d
             s2d[oracle_par_pos(nD, gi, gj)] =
                 oracle_entropy2d(t1, values + (size_t)gi * N, vmin[gi], vmax[gi], bins[3 + t1], t2,
                                  values + (size_t)gj * N, vmin[gj], vmax[gj], bins[3 + t2], N, nullptr);
+        }
+    }
+    return 0;
+}
+
+// The reference's schedule on one rank (PARENT.cpp:774-1479 with one MPI
+// process): the row dofs one after another in global order, and for each row
+// one OpenMP parallel region whose threads split that row's partners of each
+// type with the default (static) schedule and share nothing but the inputs;
+// every thread fills one histogram buffer of B_row x biggest long longs per
+// row (PARENT.cpp:786-792).  Same results as oracle_compute (pairs are
+// independent); this is the form the CPU baseline times.
+int oracle_compute_rowwise(int nD, int N, double* values, const int bins[6], double* s1d, double* s2d,
+                           int nthreads) {
+    const int D = 3 * nD + 3;
+    const double pi = acos(-1);  // PARENT.cpp:112
+    std::vector<double> vmin(D), vmax(D), mf(D);
+    int err = 0;
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for reduction(max : err)
+    for (int g = 0; g < D; g++) {
+        int r = oracle_dof_range(type_of(nD, g), N, values + (size_t)g * N, pi, &vmin[g], &vmax[g], &mf[g]);
+        if (r > err) err = r;
+    }
+    if (err) return err;
+#pragma omp parallel for
+    for (int g = 0; g < D; g++) {
+        int t = type_of(nD, g);
+        s1d[g] = oracle_entropy1d(t, N, values + (size_t)g * N, vmin[g], vmax[g], bins[t]);
+    }
+    // every value's index at its type's 2D bin count, checked once here so
+    // that the pair loop needs no test (the reference has none)
+    int bad = 0;
+#pragma omp parallel for reduction(max : bad)
+    for (int g = 0; g < D; g++) {
+        const int t = type_of(nD, g), nb = bins[3 + t];
+        const double* x = values + (size_t)g * N;
+        const double binsize = (vmax[g] - vmin[g]) / nb;
+        for (int i = 0; i < N; i++) {
+            const int k = int((x[i] - vmin[g]) / binsize);
+            if (k < 0 || k >= nb) bad = 1;
+        }
+    }
+    if (bad) return 2;
+    std::vector<const double*> rows(D);
+    for (int g = 0; g < D; g++) rows[g] = values + (size_t)g * N;
+    int biggest = 1;
+    for (int t = 0; t < 3; t++) biggest = bins[3 + t] > biggest ? bins[3 + t] : biggest;
+    for (int gi = 0; gi < D - 1; gi++) {
+        const int t1 = type_of(nD, gi);
+#pragma omp parallel
+        {
+            std::vector<long long> histo((size_t)bins[3 + t1] * biggest);
+            long long* h = histo.data();
+            for (int t2 = t1; t2 < 3; t2++) {
+                const int lo = t2 == t1 ? gi + 1 : first_of_type(nD, t2);
+                const int hi = first_of_type(nD, t2) + n_of_type(nD, t2);
+#pragma omp for
+                for (int gj = lo; gj < hi; gj++)
+                    s2d[oracle_par_pos(nD, gi, gj)] =
+                        entropy2d_core<false>(t1, rows.data(), vmin.data(), vmax.data(), gi, bins[3 + t1], t2, gj,
+                                              bins[3 + t2], N, h, biggest, nullptr);
+            }
         }
     }
     return 0;

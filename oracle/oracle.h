@@ -77,6 +77,14 @@ int oracle_compute(int n_dihedrals, int n_frames, double* values, const int bins
                    double* s1d, double* s2d, double* vmin, double* vmax,
                    double* modfit, int nthreads);
 
+/* The same computation with the reference's OpenMP schedule (rows one after
+ * another, each row's partners split statically among the threads, one
+ * histogram buffer per thread and row; PARENT.cpp:774-1479 on one rank):
+ * the form the CPU baseline times.  Same outputs as oracle_compute; returns
+ * 2 when a value falls outside its histogram. */
+int oracle_compute_rowwise(int n_dihedrals, int n_frames, double* values, const int bins[6],
+                           double* s1d, double* s2d, int nthreads);
+
 /* The same computation restricted to a selection of dofs and a list of
  * pairs among them, for problems too large to run every pair on the host.
  * cols: dof-major [n_sel][N] FP64 values of the selected global dofs

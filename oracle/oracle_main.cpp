@@ -164,7 +164,7 @@ int main(int argc, char* argv[]) {
     const int D = 3 * nD + 3;
     const long long P = (long long)D * (D - 1) / 2;
     std::vector<double> s1d(D), s2d(P);
-    int rc = oracle_compute(nD, N, v, bins, s1d.data(), s2d.data(), nullptr, nullptr, nullptr, 0);
+    int rc = oracle_compute_rowwise(nD, N, v, bins, s1d.data(), s2d.data(), 0);
     oracle_free(v);
     if (rc == 1) {
         std::cerr << "ERROR: Bond is smaller than 0.0" << std::endl;

@@ -147,6 +147,7 @@ struct parent_batch {
     std::vector<std::unique_ptr<Item>> items;
     DevBuf queue, descs;
     int queue_items = -1, queue_method = -1, queue_variant = -1, ntiles = 0;
+    const void* queue_nlogn = nullptr;  // the n ln n table the uploaded descriptors point to
     int method = PARENT_METHOD_PLAIN, variant = 0;
     int tuned_items = -1, tuned_method = -1, tuned_variant = -1;  // batch autotuner's choice
 };
@@ -1250,6 +1251,10 @@ int parent_batch_add(parent_batch* b, int n_dihedrals, int n_frames, int double_
         t.values = t.values_own.p;
     }
     memcpy(it->set.bins, b->bins, sizeof(b->bins));
+    // run_prepare may reallocate the shared n ln n table (a longer
+    // trajectory), which the uploaded queue descriptors point to: drop the
+    // queue before, so that a rejected trajectory leaves no stale pointer
+    b->queue_items = -1;
     const int rc = run_prepare(e, t, it->set);
     // the index matrix and per-dof constants are all the pairs need: the
     // uploaded values go right away
@@ -1258,7 +1263,6 @@ int parent_batch_add(parent_batch* b, int n_dihedrals, int n_frames, int double_
     if (rc) return rc;
     if (index) *index = (int)b->items.size();
     b->items.push_back(std::move(it));
-    b->queue_items = -1;
     return PARENT_OK;
 }
 
@@ -1304,7 +1308,8 @@ int parent_batch_run(parent_batch* b, int method) {
     }
     int tr, tc;
     tile_shape(sb, used, variant, &tr, &tc);
-    if (b->queue_items != n || b->queue_method != used || b->queue_variant != variant) {
+    if (b->queue_items != n || b->queue_method != used || b->queue_variant != variant ||
+        b->queue_nlogn != e->scratch.nlogn.p) {
         // Merged queue: items by decreasing frame count (cost per tile), each
         // item's tiles in its band order, padded with empty tiles to a
         // multiple of 8 and dealt so that block b of the launch runs entry b
@@ -1350,6 +1355,7 @@ int parent_batch_run(parent_batch* b, int method) {
         HIPCHK(e, hipMemcpyAsync(b->descs.p, d.data(), d.size() * sizeof(PairInputs), hipMemcpyHostToDevice, e->stream));
         HIPCHK(e, hipStreamSynchronize(e->stream));
         b->queue_items = n;
+        b->queue_nlogn = e->scratch.nlogn.p;
         b->queue_method = used;
         b->queue_variant = variant;
         b->ntiles = T;

@@ -119,7 +119,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PERM ? (UPT
         // the dof's bin-index row per chunk, stored at its slot's row of the
         // side's staging block.  Dof slots past the last dof read dof D-1
         // (rows no epilogue or dump reads); units past 2 * TD * 8 load the
-        // row dof's first group and store nothing.
+        // row dof's first group and store nothing.  Every dof is clamped to
+        // D-1, so the empty padding tiles of a batch queue (r0 = c0 = D)
+        // read inside the index matrix too.
         const int side_bytes = TD * kIdxStride, buf_bytes = 2 * side_bytes;
         const int nunits = 2 * TD * 8;
         int ubase[UPT];
@@ -129,7 +131,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PERM ? (UPT
             const int u = tid + k * 256;
             const int side = u / (TD * 8), rem = u % (TD * 8), d = rem / 8, g = rem % 8;
             const bool act = u < nunits;
-            const int dof = act ? min((side ? c0 : r0) + d, D - 1) : r0;
+            const int dof = min(act ? (side ? c0 : r0) + d : r0, D - 1);
             usrc[k] = in.bins + (size_t)dof * in.Np + (act ? g * 16 : 0);
             ubase[k] = act ? side * side_bytes + d * kIdxStride + g * 16 : -1;
         }
@@ -239,7 +241,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PERM ? (UPT
             const int u = tid + k * 256;
             const int side = u / (TD * 8), rem = u % (TD * 8), d = rem / 8, g = rem % 8;
             uact[k] = u < nunits;
-            const int dof = uact[k] ? min((side ? c0 : r0) + d, D - 1) : r0;
+            const int dof = min(uact[k] ? (side ? c0 : r0) + d : r0, D - 1);
             usrc[k] = in.bins + (size_t)dof * in.Np + g * 16;
             ubase[k] = uact[k] ? (side ? kTile * kStride : 0) + d * B * kStride + g * 4 : 0;
         }

@@ -163,3 +163,26 @@ def test_queue_longer_than_one_launch():
         assert eng.stats()["method"] == pa.METHOD_ATOMIC and eng.stats()["kernel_variant"] == 19
         s2 = eng.s2d()
         assert np.max(np.abs(s2 - s2p)) <= S_ABS
+
+
+def test_batch_rerun_after_rejected_longer_trajectory():
+    """A rejected add of a trajectory with more frames than any accepted one
+    may grow the shared n ln n table; the next run must rebuild the queue
+    descriptors that point to it and give the first run's S2D bit for bit."""
+    pa = _pa()
+    good = [pa.synth_values(12, 900, 3), pa.synth_values(9, 1200, 4)]
+    bad = pa.synth_values(10, 20000, 5)
+    bad[11, 2] = -0.25  # a negative bond: PARENT_E_DATA
+    with pa.Batch([30] * 6) as b:
+        for v in good:
+            b.add(v)
+        b.run(pa.METHOD_ATOMIC)
+        first = [b.results(k)[1].copy() for k in range(len(good))]
+        print("batch: adding the rejected 20,000-frame trajectory", flush=True)
+        with pytest.raises(pa.ParentError) as ei:
+            b.add(bad)
+        assert ei.value.code == 4
+        b.run(pa.METHOD_ATOMIC)
+        for k in range(len(good)):
+            s2 = b.results(k)[1]
+            assert np.array_equal(s2.view(np.uint64), first[k].view(np.uint64)), f"trajectory {k}"
