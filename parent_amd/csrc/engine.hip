@@ -94,13 +94,18 @@ struct BinSet {
     int method = PARENT_METHOD_PLAIN, variant = 0;  // kernel of the last pair pass
     DevBuf flags, vmin, vmax, modfit, bins2, bins1, s1d, c2, jtab, s2d;
     TileList tiles;
+    // the set whose vmin/vmax/modfit this one uses: itself (nullptr), or, in a
+    // sweep, the first set over the same trajectory (the ranges and the
+    // dihedral shift do not depend on the bin counts, PARENT.cpp:503-637)
+    const BinSet* ranges_from = nullptr;
+    const BinSet& ranges() const { return ranges_from ? *ranges_from : *this; }
 };
 
 // Scratch of the prepare and MIST kernels, shared by the bin sets of a stream
 // (they run one after another on it).
 struct Scratch {
     DevBuf emin, emax, mfhist, nlogn;
-    DevBuf mkey, mupar, minpos, mstate, medges, mmi, mleft, mflags;
+    DevBuf mdense, morder, mkey, mupos, mlive, medges, mmi, mleft, mflags;
 };
 
 }  // namespace
@@ -197,8 +202,8 @@ PairInputs pair_inputs(const Traj& t, BinSet& s) {
     in.N = t.N;
     in.Np = t.Np;
     in.B = bins2d(s);
-    in.vmin = s.vmin.as<double>();
-    in.vmax = s.vmax.as<double>();
+    in.vmin = s.ranges().vmin.as<double>();
+    in.vmax = s.ranges().vmax.as<double>();
     in.jtab = s.jtab.as<double>();
     in.c2 = s.c2.as<double>();
     in.logN = log((double)t.N);
@@ -216,9 +221,15 @@ int alloc_set(parent_engine* e, const Traj& t, BinSet& s) {
     HIPCHK(e, x.mfhist.ensure((size_t)nD * kModfitBins * 4));
     HIPCHK(e, x.nlogn.ensure(((size_t)t.N + 1) * 8));
     HIPCHK(e, s.flags.ensure(64));
-    HIPCHK(e, s.vmin.ensure(D * 8));
-    HIPCHK(e, s.vmax.ensure(D * 8));
-    HIPCHK(e, s.modfit.ensure(D * 8));
+    if (s.ranges_from) {
+        s.vmin.release();
+        s.vmax.release();
+        s.modfit.release();
+    } else {
+        HIPCHK(e, s.vmin.ensure(D * 8));
+        HIPCHK(e, s.vmax.ensure(D * 8));
+        HIPCHK(e, s.modfit.ensure(D * 8));
+    }
     HIPCHK(e, s.bins2.ensure((size_t)D * t.Np));
     if (need1) HIPCHK(e, s.bins1.ensure((size_t)D * t.Np));
     else s.bins1.release();
@@ -231,10 +242,12 @@ int alloc_set(parent_engine* e, const Traj& t, BinSet& s) {
 
 // Ranges, modfit, pre-binning and 1D entropies of set s: kernels only, no
 // host synchronisation (capturable).  The n ln n table of t.N goes to the
-// shared scratch.
+// shared scratch.  A set that takes its ranges from another one runs only
+// the pre-binning and 1D kernels.
 hipError_t enqueue_prepare(parent_engine* e, const Traj& t, BinSet& s) {
     const bool need1 = s.bins1.p != nullptr;
     PrepareArgs a{};
+    a.stages = s.ranges_from ? kPrepareBins : (kPrepareRanges | kPrepareBins | kPrepareNlogn);
     a.values = t.values;
     a.double_prec = t.dp;
     a.nD = t.nD;
@@ -248,9 +261,9 @@ hipError_t enqueue_prepare(parent_engine* e, const Traj& t, BinSet& s) {
     a.emax = e->scratch.emax.as<unsigned long long>();
     a.mfhist = e->scratch.mfhist.as<unsigned>();
     a.flags = s.flags.as<int>();
-    a.vmin = s.vmin.as<double>();
-    a.vmax = s.vmax.as<double>();
-    a.modfit = s.modfit.as<double>();
+    a.vmin = const_cast<double*>(s.ranges().vmin.as<double>());
+    a.vmax = const_cast<double*>(s.ranges().vmax.as<double>());
+    a.modfit = const_cast<double*>(s.ranges().modfit.as<double>());
     a.bins2 = s.bins2.as<uint8_t>();
     a.bins1 = need1 ? s.bins1.as<uint8_t>() : nullptr;
     a.s1d = s.s1d.as<double>();
@@ -591,30 +604,39 @@ int run_mist(parent_engine* e, BinSet& s, int32_t* edges, double* mi, int32_t* l
     if (!s.s2d_valid || s.ent_nD < 1) return fail(e, PARENT_E_INVALID, "no 2D entropies on the device");
     const int nD = s.ent_nD, V = n_dofs(nD);
     Scratch& x = e->scratch;
-    HIPCHK(e, x.mkey.ensure(V * 8));
-    HIPCHK(e, x.mupar.ensure(V * 4));
-    HIPCHK(e, x.minpos.ensure(V * 4));
-    HIPCHK(e, x.mstate.ensure(V * 4));
+    // the dense MI matrix: each Prim step then reads one contiguous row
+    // (c2: 162 MB, c5: 1.8 GB); without room for it the kernel gathers MI
+    // from S1D / S2D
+    const size_t dense_bytes = (size_t)V * V * 8;
+    bool dense = x.mdense.ensure(dense_bytes) == hipSuccess;
+    if (!dense) (void)hipGetLastError();
+    HIPCHK(e, x.morder.ensure((size_t)V * 4));
     HIPCHK(e, x.medges.ensure((size_t)V * 16));
-    HIPCHK(e, x.mmi.ensure(V * 8));
-    HIPCHK(e, x.mleft.ensure(V * 4));
+    HIPCHK(e, x.mmi.ensure((size_t)V * 8));
+    HIPCHK(e, x.mleft.ensure((size_t)V * 4));
     HIPCHK(e, x.mflags.ensure(64));
     MistArgs a{};
     a.nD = nD;
     a.V = V;
     a.s1d = s.s1d.as<double>();
     a.s2d = s.s2d.as<double>();
-    a.key = x.mkey.as<double>();
-    a.upar = x.mupar.as<int>();
-    a.inpos = x.minpos.as<int>();
-    a.state = x.mstate.as<int>();
+    a.mi_dense = dense ? x.mdense.as<double>() : nullptr;
+    a.order = x.morder.as<int>();
+    a.flags = x.mflags.as<int>();
     a.edges = x.medges.as<int>();
     a.mi = x.mmi.as<double>();
     a.left = x.mleft.as<int>();
-    a.flags = x.mflags.as<int>();
+    if (mist_keys_in_memory(V, dense)) {
+        HIPCHK(e, x.mkey.ensure((size_t)V * 8));
+        HIPCHK(e, x.mupos.ensure((size_t)V * 4));
+        HIPCHK(e, x.mlive.ensure((size_t)V * 4));
+        a.key = x.mkey.as<double>();
+        a.upos = x.mupos.as<unsigned>();
+        a.live = x.mlive.as<int>();
+    }
     int flags = 0;
-    HIPCHK(e, hipMemcpyAsync(x.mflags.p, &flags, sizeof(int), hipMemcpyHostToDevice, e->stream));
     HIPCHK(e, hipEventRecord(e->ev[6], e->stream));
+    if (dense) HIPCHK(e, launch_mi_dense(nD, a.s1d, a.s2d, x.mdense.as<double>(), e->stream));
     HIPCHK(e, launch_mist(a, e->stream));
     HIPCHK(e, hipEventRecord(e->ev[7], e->stream));
     std::vector<double> s1(V);
@@ -928,6 +950,7 @@ int parent_engine_prepare(parent_engine* e, const int bins[6]) {
     e->sets.resize(1);
     e->active = 0;
     BinSet& s = *e->sets[0];
+    s.ranges_from = nullptr;
     memcpy(s.bins, bins, sizeof(s.bins));
     return run_prepare(e, e->traj, s);
 }
@@ -959,6 +982,7 @@ int parent_engine_sweep(parent_engine* e, int n_sets, const int* bins, int metho
     if (!e->graph_exec) {
         for (int k = 0; k < n_sets; k++) {
             BinSet& s = *e->sets[k];
+            s.ranges_from = k ? e->sets[0].get() : nullptr;
             memcpy(s.bins, bins + 6 * k, sizeof(s.bins));
             if (int rc = run_prepare(e, t, s)) return rc;
             if (int rc = plan_pairs(e, t, s, method)) return rc;
@@ -1026,9 +1050,10 @@ int parent_engine_read_ranges(parent_engine* e, double* vmin, double* vmax, doub
     if (!s.prepared) return fail(e, PARENT_E_INVALID, "parent_engine_prepare has not succeeded");
     HIPCHK(e, hipSetDevice(e->device));
     const size_t D = e->traj.D;
-    if (vmin) HIPCHK(e, hipMemcpyAsync(vmin, s.vmin.p, D * 8, hipMemcpyDeviceToHost, e->stream));
-    if (vmax) HIPCHK(e, hipMemcpyAsync(vmax, s.vmax.p, D * 8, hipMemcpyDeviceToHost, e->stream));
-    if (modfit) HIPCHK(e, hipMemcpyAsync(modfit, s.modfit.p, D * 8, hipMemcpyDeviceToHost, e->stream));
+    const BinSet& r = s.ranges();
+    if (vmin) HIPCHK(e, hipMemcpyAsync(vmin, r.vmin.p, D * 8, hipMemcpyDeviceToHost, e->stream));
+    if (vmax) HIPCHK(e, hipMemcpyAsync(vmax, r.vmax.p, D * 8, hipMemcpyDeviceToHost, e->stream));
+    if (modfit) HIPCHK(e, hipMemcpyAsync(modfit, r.modfit.p, D * 8, hipMemcpyDeviceToHost, e->stream));
     HIPCHK(e, hipStreamSynchronize(e->stream));
     return PARENT_OK;
 }

@@ -6,7 +6,12 @@
 
 namespace parent {
 
+// stages of launch_prepare: the ranges and dihedral shifts (scans), the bin
+// indices and 1D statistics (which read vmin/vmax/modfit), the n ln n table
+enum : int { kPrepareRanges = 1, kPrepareBins = 2, kPrepareNlogn = 4 };
+
 struct PrepareArgs {
+    int stages;          // kPrepare* bits
     const void* values;  // frame-major [N][D]
     int double_prec, nD, D, N, Np;
     double pi;
@@ -105,13 +110,22 @@ hipError_t launch_pairs_mfma(const PairInputs& in, int bmax, const TileQueue& q,
 struct MistArgs {
     int nD, V;
     const double *s1d, *s2d;
-    double* key;
-    int *upar, *inpos, *state;
-    int* edges;
-    double* mi;
-    int* left;
-    int* flags;
+    const double* mi_dense;  // [V][V] MI matrix (launch_mi_dense), or nullptr: MI gathered from s1d / s2d
+    int* order;              // [V] scratch: the vertex inserted at each step (keys in memory only)
+    int* flags;              // [1] scratch: kFlagMist when no candidate is left
+    int* edges;              // [V-1][4] outputs, as parent_engine_mist
+    double* mi;              // [V-1]
+    int* left;               // [V-1]
+    double* key;             // [V] scratch, only when mist_keys_in_memory(V, mi_dense != nullptr)
+    unsigned* upos;          // [V] idem
+    int* live;               // [V] idem
 };
+// vertices whose keys one workgroup keeps in registers
+int mist_reg_max_vertices();
+// whether launch_mist needs MistArgs key/upos/live (V above the register form's limit)
+bool mist_keys_in_memory(int V, bool dense);
 hipError_t launch_mist(const MistArgs& a, hipStream_t s);
+// dense MI matrix out[V][V] of the .par entropies (diagonal 0)
+hipError_t launch_mi_dense(int nD, const double* s1d, const double* s2d, double* out, hipStream_t s);
 
 }  // namespace parent

@@ -477,40 +477,44 @@ hipError_t launch_prepare(const PrepareArgs& a, hipStream_t s) {
     const int nDih = nD;
     const double pi = a.pi, twopi = 2 * pi;
     const double mf_bin = (2 * pi + 0.000000005) / kModfitBins;
-    k_fill_u64<<<64, 256, 0, s>>>(a.emin, (size_t)D, ~0ull);
-    k_fill_u64<<<64, 256, 0, s>>>(a.emax, (size_t)D, 0ull);
-    if (nDih > 0) k_fill_u32<<<64, 256, 0, s>>>(a.mfhist, (size_t)nDih * kModfitBins, 0u);
     k_fill_u32<<<1, 32, 0, s>>>((unsigned*)a.flags, 1, 0u);
-    const int fpb = 2048;
-    const int nbx1 = (D + 63) / 64, nby1 = (N + fpb - 1) / fpb;
-    const unsigned g1 = xcd_grid(nbx1, nby1);
-    if (a.double_prec)
-        k_scan_raw<double><<<g1, 256, 0, s>>>((const double*)a.values, N, D, nD, fpb, mf_bin, nbx1, nby1, a.emin, a.emax, a.mfhist, a.flags);
-    else
-        k_scan_raw<float><<<g1, 256, 0, s>>>((const float*)a.values, N, D, nD, fpb, mf_bin, nbx1, nby1, a.emin, a.emax, a.mfhist, a.flags);
-    k_finalize_ranges<<<(D + kFinalizeBlock - 1) / kFinalizeBlock, kFinalizeBlock, 0, s>>>(D, nD, pi, mf_bin, a.emin, a.emax, a.mfhist, a.vmin, a.vmax, a.modfit, a.flags);
-    if (nDih > 0) {
-        const int nbx3 = (nDih + 63) / 64, nby3 = (N + fpb - 1) / fpb;
-        const unsigned g3 = xcd_grid(nbx3, nby3);
+    if (a.stages & kPrepareRanges) {
+        k_fill_u64<<<64, 256, 0, s>>>(a.emin, (size_t)D, ~0ull);
+        k_fill_u64<<<64, 256, 0, s>>>(a.emax, (size_t)D, 0ull);
+        if (nDih > 0) k_fill_u32<<<64, 256, 0, s>>>(a.mfhist, (size_t)nDih * kModfitBins, 0u);
+        const int fpb = 2048;
+        const int nbx1 = (D + 63) / 64, nby1 = (N + fpb - 1) / fpb;
+        const unsigned g1 = xcd_grid(nbx1, nby1);
         if (a.double_prec)
-            k_scan_dihedral<double><<<g3, 256, 0, s>>>((const double*)a.values, N, D, nD, fpb, twopi, nbx3, nby3, a.modfit, a.emin, a.emax);
+            k_scan_raw<double><<<g1, 256, 0, s>>>((const double*)a.values, N, D, nD, fpb, mf_bin, nbx1, nby1, a.emin, a.emax, a.mfhist, a.flags);
         else
-            k_scan_dihedral<float><<<g3, 256, 0, s>>>((const float*)a.values, N, D, nD, fpb, twopi, nbx3, nby3, a.modfit, a.emin, a.emax);
-        k_finalize_dihedral<<<(nDih + 255) / 256, 256, 0, s>>>(D, nD, twopi, a.emin, a.emax, a.vmin, a.vmax);
+            k_scan_raw<float><<<g1, 256, 0, s>>>((const float*)a.values, N, D, nD, fpb, mf_bin, nbx1, nby1, a.emin, a.emax, a.mfhist, a.flags);
+        k_finalize_ranges<<<(D + kFinalizeBlock - 1) / kFinalizeBlock, kFinalizeBlock, 0, s>>>(D, nD, pi, mf_bin, a.emin, a.emax, a.mfhist, a.vmin, a.vmax, a.modfit, a.flags);
+        if (nDih > 0) {
+            const int nbx3 = (nDih + 63) / 64, nby3 = (N + fpb - 1) / fpb;
+            const unsigned g3 = xcd_grid(nbx3, nby3);
+            if (a.double_prec)
+                k_scan_dihedral<double><<<g3, 256, 0, s>>>((const double*)a.values, N, D, nD, fpb, twopi, nbx3, nby3, a.modfit, a.emin, a.emax);
+            else
+                k_scan_dihedral<float><<<g3, 256, 0, s>>>((const float*)a.values, N, D, nD, fpb, twopi, nbx3, nby3, a.modfit, a.emin, a.emax);
+            k_finalize_dihedral<<<(nDih + 255) / 256, 256, 0, s>>>(D, nD, twopi, a.emin, a.emax, a.vmin, a.vmax);
+        }
     }
-    const int nbx5 = (D + 63) / 64, nby5 = a.Np / 256;
-    const unsigned g5 = xcd_grid(nbx5, nby5);
-    const bool need1 = a.bins1 != nullptr;
-    if (a.double_prec && need1)
-        k_prebin<double, true><<<g5, 256, 0, s>>>((const double*)a.values, N, a.Np, D, nD, a.vmin, a.vmax, a.modfit, twopi, a.B2, a.B1, nbx5, nby5, a.bins2, a.bins1, a.flags);
-    else if (a.double_prec)
-        k_prebin<double, false><<<g5, 256, 0, s>>>((const double*)a.values, N, a.Np, D, nD, a.vmin, a.vmax, a.modfit, twopi, a.B2, a.B1, nbx5, nby5, a.bins2, a.bins1, a.flags);
-    else if (need1)
-        k_prebin<float, true><<<g5, 256, 0, s>>>((const float*)a.values, N, a.Np, D, nD, a.vmin, a.vmax, a.modfit, twopi, a.B2, a.B1, nbx5, nby5, a.bins2, a.bins1, a.flags);
-    else
-        k_prebin<float, false><<<g5, 256, 0, s>>>((const float*)a.values, N, a.Np, D, nD, a.vmin, a.vmax, a.modfit, twopi, a.B2, a.B1, nbx5, nby5, a.bins2, a.bins1, a.flags);
-    k_dof_stats<<<D, 256, 0, s>>>(a.bins2, a.bins1, N, a.Np, nD, a.B1, a.B2, a.vmin, a.vmax, a.s1d, a.c2, a.jtab);
-    k_nlogn<<<(N + 256) / 256, 256, 0, s>>>(N, a.nlogn);
+    if (a.stages & kPrepareBins) {
+        const int nbx5 = (D + 63) / 64, nby5 = a.Np / 256;
+        const unsigned g5 = xcd_grid(nbx5, nby5);
+        const bool need1 = a.bins1 != nullptr;
+        if (a.double_prec && need1)
+            k_prebin<double, true><<<g5, 256, 0, s>>>((const double*)a.values, N, a.Np, D, nD, a.vmin, a.vmax, a.modfit, twopi, a.B2, a.B1, nbx5, nby5, a.bins2, a.bins1, a.flags);
+        else if (a.double_prec)
+            k_prebin<double, false><<<g5, 256, 0, s>>>((const double*)a.values, N, a.Np, D, nD, a.vmin, a.vmax, a.modfit, twopi, a.B2, a.B1, nbx5, nby5, a.bins2, a.bins1, a.flags);
+        else if (need1)
+            k_prebin<float, true><<<g5, 256, 0, s>>>((const float*)a.values, N, a.Np, D, nD, a.vmin, a.vmax, a.modfit, twopi, a.B2, a.B1, nbx5, nby5, a.bins2, a.bins1, a.flags);
+        else
+            k_prebin<float, false><<<g5, 256, 0, s>>>((const float*)a.values, N, a.Np, D, nD, a.vmin, a.vmax, a.modfit, twopi, a.B2, a.B1, nbx5, nby5, a.bins2, a.bins1, a.flags);
+        k_dof_stats<<<D, 256, 0, s>>>(a.bins2, a.bins1, N, a.Np, nD, a.B1, a.B2, a.vmin, a.vmax, a.s1d, a.c2, a.jtab);
+    }
+    if (a.stages & kPrepareNlogn) k_nlogn<<<(N + 256) / 256, 256, 0, s>>>(N, a.nlogn);
     return hipGetLastError();
 }
 

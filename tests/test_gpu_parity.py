@@ -330,6 +330,24 @@ def test_mist_ties_follow_reference_scan(engine):
     assert np.array_equal(tot, to)
 
 
+def test_mist_ties_between_signed_zeros(engine):
+    """MI values of exactly +0 and -0 (S1D of mixed signed zeros): the
+    reference's '>' treats them as equal, so the scan order decides, and each
+    edge's MI keeps the bits of the pair that won."""
+    nD = 7
+    D = 3 * nD + 3
+    rng = np.random.default_rng(4)
+    s1 = np.where(rng.integers(0, 2, D) == 1, -0.0, 0.0)
+    P = D * (D - 1) // 2
+    s2 = -rng.integers(0, 2, P).astype(np.float64)  # MI in {+-0, 1}
+    engine.set_entropies(nD, s1, s2)
+    edges, mi, left, tot = engine.mist()
+    eo, mio, lo, to = O.mist(nD, s1, s2)
+    assert np.array_equal(edges, eo)
+    assert np.array_equal(mi.view(np.uint64), mio.view(np.uint64))
+    assert np.array_equal(tot.view(np.uint64), to.view(np.uint64))
+
+
 def test_drop_in_executables(tmp_path):
     """PARENT_hip.x / get_PAR_MIST_hip.x against the oracle's PARENT.x restatement."""
     pa = _pa()
