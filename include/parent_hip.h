@@ -242,6 +242,25 @@ int parent_engine_pair_counts(parent_engine* eng, int method, int n_pairs, const
 int parent_engine_mist(parent_engine* eng, int32_t* edges, double* mi, int32_t* left,
                        double* totals);
 
+/* MIST of entropies already in device memory, allocating nothing: the same
+ * Prim as parent_engine_mist (edges, MI bits, left counts and totals are
+ * identical), with its scratch in a static device array of 1.25 MiB per GPU
+ * instead of engine buffers.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops).
+ *   s1d [D], s2d [D(D-1)/2] (.par body order): inputs in device memory
+ *   edges [(D-1) x 4], mi [D-1], left [D-1], totals [PARENT_TOT_COUNT],
+ *   status [1]: outputs in device memory; status = 0, or PARENT_E_DATA when
+ *   no finite MI was left to pick (the edges are then incomplete)
+ *   stream: a hipStream_t of the engine's device, or NULL for the engine's
+ * The kernels are enqueued on the stream and the call returns without
+ * waiting for them (nothing synchronises).  D may be at most 65,536
+ * (PARENT_E_UNSUPPORTED above).  Every call on one GPU shares that GPU's
+ * scratch array: calls that overlap on one GPU, from any engine, race. */
+int parent_engine_mist_noalloc(parent_engine* eng, int n_dihedrals, const double* s1d, const double* s2d,
+                               int32_t* edges, double* mi, int32_t* left, double* totals, int32_t* status,
+                               void* stream);
+
 /* Loads S1D/S2D from host arrays (e.g. a .par read with parent_par_read) so
  * parent_engine_mist can run without a trajectory (the get_PAR_MIST.x path,
  * get_PAR_MIST.cpp:211-327). */

@@ -59,6 +59,7 @@ def lib() -> C.CDLL:
         "parent_engine_pair_counts": (i32, [p, i32, i32, p, p]),
         "parent_engine_mist": (i32, [p, p, dp, p, dp]),
         "parent_engine_set_entropies": (i32, [p, i32, dp, dp]),
+        "parent_engine_mist_noalloc": (i32, [p, i32, p, p, p, p, p, p, p, p]),
         "parent_engine_sync": (i32, [p]),
         "parent_engine_autotune": (i32, [p, p, i32, C.POINTER(i32)]),
         "parent_engine_set_variant": (i32, [p, i32, i32]),
@@ -367,6 +368,14 @@ class Engine:
         tot = np.empty(10, dtype=np.float64)
         self._check(lib().parent_engine_mist(self._h, _ptr(edges), _ptr(mi), _ptr(left), _ptr(tot)))
         return edges, mi, left, tot
+
+    def mist_noalloc(self, n_dihedrals: int, s1d: int, s2d: int, edges: int, mi: int, left: int, totals: int,
+                     status: int, stream: int | None = None) -> None:
+        """parent_engine_mist_noalloc: every argument is a device address (int);
+        enqueued on `stream` (a hipStream_t as int, None: the engine's), not waited for."""
+        v = [C.c_void_p(x) for x in (s1d, s2d, edges, mi, left, totals, status)]
+        self._check(lib().parent_engine_mist_noalloc(self._h, n_dihedrals, *v,
+                                                     C.c_void_p(stream) if stream else None))
 
     def autotune(self) -> list:
         """Times every pair-kernel configuration on a sample; returns the table."""

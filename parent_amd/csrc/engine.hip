@@ -117,6 +117,7 @@ struct parent_engine {
     std::string err;
     int lds_max = 65536;
     int cus = 256;  // compute units, sizes the autotuner's samples
+    void* noalloc_scratch = nullptr;  // this device's static MIST scratch (device address)
 
     Traj traj;
     Scratch scratch;
@@ -815,6 +816,14 @@ int parent_engine_create(int device, parent_engine** out) {
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
         e->cus = cus;
+    // the device address of this GPU's static scratch of
+    // parent_engine_mist_noalloc (looked up once per device)
+    const hipError_t hs = mist_noalloc_scratch(device, &e->noalloc_scratch);
+    if (hs != hipSuccess) {
+        set_last_error(std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(hs));
+        parent_engine_destroy(e);
+        return PARENT_E_DEVICE;
+    }
     *out = e;
     return PARENT_OK;
 }
@@ -1170,6 +1179,19 @@ int parent_engine_mist(parent_engine* e, int32_t* edges, double* mi, int32_t* le
     if (!e || !edges || !mi || !left || !totals) return PARENT_E_INVALID;
     HIPCHK(e, hipSetDevice(e->device));
     return run_mist(e, cur(e), edges, mi, left, totals);
+}
+
+int parent_engine_mist_noalloc(parent_engine* e, int n_dihedrals, const double* s1d, const double* s2d,
+                               int32_t* edges, double* mi, int32_t* left, double* totals, int32_t* status,
+                               void* stream) {
+    if (!e || !s1d || !s2d || !edges || !mi || !left || !totals || !status) return PARENT_E_INVALID;
+    if (n_dihedrals < 1) return fail(e, PARENT_E_INVALID, "need n_dihedrals >= 1");
+    if (n_dofs(n_dihedrals) > kMistNoAllocMaxVertices)
+        return fail(e, PARENT_E_UNSUPPORTED, "parent_engine_mist_noalloc: more than 65,536 dofs");
+    HIPCHK(e, hipSetDevice(e->device));
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : e->stream;
+    HIPCHK(e, launch_mist_noalloc(n_dihedrals, s1d, s2d, edges, mi, left, totals, status, e->noalloc_scratch, s));
+    return PARENT_OK;
 }
 
 int parent_engine_autotune(parent_engine* e, parent_tune_entry* entries, int max_entries, int* n_entries) {

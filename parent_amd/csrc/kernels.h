@@ -127,5 +127,17 @@ bool mist_keys_in_memory(int V, bool dense);
 hipError_t launch_mist(const MistArgs& a, hipStream_t s);
 // dense MI matrix out[V][V] of the .par entropies (diagonal 0)
 hipError_t launch_mi_dense(int nD, const double* s1d, const double* s2d, double* out, hipStream_t s);
+// MIST with its scratch in a static device array (no allocation), up to
+// kMistNoAllocMaxVertices vertices; scratch_dev = mist_noalloc_scratch's
+// address for the current device.  Outputs as launch_mist, plus the totals
+// and status of launch_mist_totals.
+constexpr int kMistNoAllocMaxVertices = 65536;
+size_t mist_noalloc_scratch_bytes();
+hipError_t mist_noalloc_scratch(int device, void** out);
+hipError_t launch_mist_noalloc(int nD, const double* s1d, const double* s2d, int* edges, double* mi, int* left,
+                               double* totals, int* status, void* scratch_dev, hipStream_t s);
+// MIST totals (PARENT_TOT_* order) and status (0, or PARENT_E_DATA) on the device
+hipError_t launch_mist_totals(int nD, const double* s1d, const int* edges, const double* mi, const int* flags,
+                              double* totals, int* status, hipStream_t s);
 
 }  // namespace parent

@@ -39,6 +39,9 @@
 // One barrier per step; the only per-step memory traffic is row w.
 #include "kernels.h"
 
+#include <map>
+#include <mutex>
+
 namespace parent {
 
 namespace {
@@ -372,6 +375,51 @@ __global__ __launch_bounds__(256) void k_mi_dense(int nD, int V, const double* _
     }
 }
 
+// MIST totals in the reference's order (get_PAR_MIST.cpp:276-284, 663-741,
+// 754-765): per-type S1D sums, per-category tree MI sums in step order, and
+// total = sum S1D - sum tree MI.  One workgroup; the values are staged in LDS
+// 1,024 at a time and thread 0 adds them in sequence.  status = 0, or
+// PARENT_E_DATA (4) when the tree could not be completed.
+__global__ __launch_bounds__(1024) void k_mist_totals(int nD, const double* __restrict__ s1d,
+                                                      const int* __restrict__ edges, const double* __restrict__ mi,
+                                                      const int* __restrict__ flags, double* totals, int* status) {
+    __shared__ double buf[1024];
+    __shared__ int cat[1024];
+    const int V = 3 * nD + 3, tid = threadIdx.x;
+    double t1[3] = {0, 0, 0}, c6[6] = {0, 0, 0, 0, 0, 0}, tmi = 0;
+    for (int t = 0; t < 3; t++) {
+        const int f0 = dof_first(nD, t), n = t == 0 ? nD + 2 : (t == 1 ? nD + 1 : nD);
+        for (int b0 = 0; b0 < n; b0 += 1024) {
+            __syncthreads();
+            if (b0 + tid < n) buf[tid] = s1d[f0 + b0 + tid];
+            __syncthreads();
+            if (tid == 0)
+                for (int i = 0; i < min(1024, n - b0); i++) t1[t] += buf[i];
+        }
+    }
+    const bool ok = flags[0] == 0;
+    for (int b0 = 0; ok && b0 < V - 1; b0 += 1024) {
+        __syncthreads();
+        if (b0 + tid < V - 1) {
+            const int ta = edges[4 * (b0 + tid)], tb = edges[4 * (b0 + tid) + 1];
+            const int lo = min(ta, tb), hi = max(ta, tb);
+            buf[tid] = mi[b0 + tid];
+            cat[tid] = lo == 0 ? hi : (lo == 1 ? 2 + hi : 5);
+        }
+        __syncthreads();
+        if (tid == 0)
+            for (int i = 0; i < min(1024, V - 1 - b0); i++) {
+                tmi += buf[i];
+                c6[cat[i]] += buf[i];
+            }
+    }
+    if (tid != 0) return;
+    for (int t = 0; t < 3; t++) totals[t] = t1[t];
+    for (int c = 0; c < 6; c++) totals[3 + c] = c6[c];
+    totals[9] = t1[0] + t1[1] + t1[2] - tmi;
+    *status = ok ? 0 : 4;
+}
+
 // K = 16 slots take up to 116 VGPRs (1,024 threads allow 128); K = 32 spills
 int mist_reg_max_vertices() { return 16 * 1024; }
 
@@ -401,6 +449,79 @@ hipError_t launch_mist(const MistArgs& a, hipStream_t s) {
     PARENT_MIST(1) PARENT_MIST(2) PARENT_MIST(4) PARENT_MIST(8) PARENT_MIST(16)
 #undef PARENT_MIST
     return hipErrorInvalidValue;
+}
+
+// ---------------------------------------------------------------------------
+// MIST without device allocations: its scratch (order, flags and, above 8,192
+// vertices, the keys) lives in one file-scope device array per GPU, whose
+// device address each GPU's first user looks up with hipGetSymbolAddress and
+// keeps.  Every call on a GPU shares that GPU's array: calls that overlap on
+// one GPU race.  The kernels initialise every scratch word they read.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int kNoAllocMaxV = kMistNoAllocMaxVertices;
+// byte offsets inside the array
+constexpr size_t kNaFlags = 0, kNaOrder = 256, kNaKey = kNaOrder + (size_t)kNoAllocMaxV * 4,
+                 kNaUpos = kNaKey + (size_t)kNoAllocMaxV * 8, kNaLive = kNaUpos + (size_t)kNoAllocMaxV * 4,
+                 kNaBytes = kNaLive + (size_t)kNoAllocMaxV * 4;
+
+}  // namespace
+
+// 1.25 MiB per GPU, held until the process ends.  In host code `scratch`
+// names a host-side stand-in: kernels get the address from
+// mist_noalloc_scratch.
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+size_t mist_noalloc_scratch_bytes() { return kNaBytes; }
+
+// The device address of this GPU's array (the caller has made `device`
+// current).  Looked up once per device and kept.
+hipError_t mist_noalloc_scratch(int device, void** out) {
+    static std::mutex mu;
+    static std::map<int, void*> addr;
+    std::lock_guard<std::mutex> g(mu);
+    auto it = addr.find(device);
+    if (it != addr.end()) {
+        *out = it->second;
+        return hipSuccess;
+    }
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    addr[device] = p;
+    *out = p;
+    return hipSuccess;
+}
+
+hipError_t launch_mist_noalloc(int nD, const double* s1d, const double* s2d, int* edges, double* mi, int* left,
+                               double* totals, int* status, void* scratch_dev, hipStream_t s) {
+    const int V = 3 * nD + 3;
+    if (V < 2 || V > kNoAllocMaxV || !scratch_dev) return hipErrorInvalidValue;
+    unsigned char* base = static_cast<unsigned char*>(scratch_dev);
+    MistArgs a{};
+    a.nD = nD;
+    a.V = V;
+    a.s1d = s1d;
+    a.s2d = s2d;
+    a.mi_dense = nullptr;
+    a.flags = reinterpret_cast<int*>(base + kNaFlags);
+    a.order = reinterpret_cast<int*>(base + kNaOrder);
+    a.key = reinterpret_cast<double*>(base + kNaKey);
+    a.upos = reinterpret_cast<unsigned*>(base + kNaUpos);
+    a.live = reinterpret_cast<int*>(base + kNaLive);
+    a.edges = edges;
+    a.mi = mi;
+    a.left = left;
+    hipError_t e = launch_mist(a, s);
+    if (e != hipSuccess) return e;
+    return launch_mist_totals(nD, s1d, edges, mi, a.flags, totals, status, s);
+}
+
+hipError_t launch_mist_totals(int nD, const double* s1d, const int* edges, const double* mi, const int* flags,
+                              double* totals, int* status, hipStream_t s) {
+    k_mist_totals<<<1, 1024, 0, s>>>(nD, s1d, edges, mi, flags, totals, status);
+    return hipGetLastError();
 }
 
 }  // namespace parent
