@@ -44,7 +44,7 @@ enum {
     PARENT_E_DATA = 4,        /* a bond length < 0 (PARENT.cpp:522-525) */
     PARENT_E_RANGE = 5,       /* a value falls outside its histogram (reference: out-of-bounds write) */
     PARENT_E_DEVICE = 6,      /* HIP runtime error */
-    PARENT_E_UNSUPPORTED = 7  /* request outside what this library implements (e.g. > 255 bins) */
+    PARENT_E_UNSUPPORTED = 7  /* request outside what this library implements (e.g. > 4096 bins) */
 };
 
 /* pair-kernel selection for parent_engine_pairs() */
@@ -52,7 +52,10 @@ enum {
     PARENT_METHOD_PLAIN = 0,  /* one dof pair per workgroup, LDS atomic histogram, per-cell FP64 epilogue */
     PARENT_METHOD_ATOMIC = 1, /* tiled: many pairs per workgroup, LDS atomic histograms, fused n*ln(n) epilogue */
     PARENT_METHOD_MFMA = 2,   /* one-hot int8 x int8 -> int32 MFMA contraction, fused epilogue */
-    PARENT_METHOD_AUTO = 3    /* autotuned choice per bin count */
+    PARENT_METHOD_AUTO = 3,   /* autotuned choice per bin count */
+    PARENT_METHOD_SLABS = 4   /* one dof pair per workgroup, the histogram in row slabs that fit LDS:
+                                 every bin count up to 4096; the kernel any request runs on when the
+                                 2D histogram does not fit LDS or a 2D bin count exceeds 255 */
 };
 
 /* MIST totals, in the order get_PAR_MIST.x prints them (get_PAR_MIST.cpp:754-765) */
@@ -189,7 +192,9 @@ int parent_engine_set_trajectory(parent_engine* eng, int n_dihedrals, int n_fram
                                  int double_prec, const void* values, int on_device);
 
 /* Extrema, dihedral modfit, bin indices and 1D entropies: PARENT.cpp:503-772
- * (rules at 511-637, 1D at 644-772).  Bins must be in [1,255]. */
+ * (rules at 511-637, 1D at 644-772).  Bins must be in [1, 4096] (the
+ * reference takes any positive count, PARENT.cpp:211-238); indices are
+ * stored as uint8 while a matrix's counts are <= 255, else as uint16. */
 int parent_engine_prepare(parent_engine* eng, const int bins[6]);
 
 /* All 2D entropies S_ij for the bins of the last prepare:
@@ -223,8 +228,12 @@ int parent_engine_read_s2d(parent_engine* eng, double* s2d /* D(D-1)/2 */);
 int parent_engine_read_ranges(parent_engine* eng, double* vmin, double* vmax, double* modfit);
 
 /* The 2D bin indices, dof-major: out[g*n_frames + f] (PARENT.cpp:803 index
- * expressions, here computed once per dof and frame). */
+ * expressions, here computed once per dof and frame).  PARENT_E_UNSUPPORTED
+ * when a 2D bin count exceeds 255: then use parent_engine_read_bins16. */
 int parent_engine_read_bins(parent_engine* eng, uint8_t* out);
+
+/* The same for any bin count, as uint16. */
+int parent_engine_read_bins16(parent_engine* eng, uint16_t* out);
 
 /* Histogram counts of listed pairs through `method`'s fill path, without the
  * epilogue: pairs[2k], pairs[2k+1] = (gi < gj); counts of pair k are
@@ -283,8 +292,10 @@ int parent_engine_autotune(parent_engine* eng, parent_tune_entry* entries, int m
 /* Pins the configuration `method` uses when called explicitly (not through
  * PARENT_METHOD_AUTO): variant in [0, count) or -1 for the default.  A
  * variant whose histograms do not fit the bins makes the method fall back to
- * PARENT_METHOD_PLAIN.  For testing every configuration the autotuner may
- * pick; no counterpart in the reference. */
+ * PARENT_METHOD_PLAIN.  For PARENT_METHOD_SLABS the variant caps the
+ * histogram rows per slab (0 or -1: as many as the LDS holds), which
+ * exercises several slabs at small bin counts.  For testing every
+ * configuration the autotuner may pick; no counterpart in the reference. */
 int parent_engine_set_variant(parent_engine* eng, int method, int variant);
 int parent_engine_variant_count(parent_engine* eng, int method, int* count);
 

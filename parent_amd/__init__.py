@@ -18,6 +18,7 @@ from .api import (  # noqa: F401
     METHOD_ATOMIC,
     METHOD_MFMA,
     METHOD_PLAIN,
+    METHOD_SLABS,
     Bat,
     Batch,
     bat_build,

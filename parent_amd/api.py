@@ -10,7 +10,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "_build", "libparent_hip.so")
 _lib = None
 
-METHOD_PLAIN, METHOD_ATOMIC, METHOD_MFMA, METHOD_AUTO = 0, 1, 2, 3
+METHOD_PLAIN, METHOD_ATOMIC, METHOD_MFMA, METHOD_AUTO, METHOD_SLABS = 0, 1, 2, 3, 4
 STATUS = {0: "OK", 1: "E_INVALID", 2: "E_NOMEM", 3: "E_IO", 4: "E_DATA", 5: "E_RANGE",
           6: "E_DEVICE", 7: "E_UNSUPPORTED"}
 TYPE_CHAR = "bad"
@@ -56,6 +56,7 @@ def lib() -> C.CDLL:
         "parent_engine_read_s2d": (i32, [p, dp]),
         "parent_engine_read_ranges": (i32, [p, dp, dp, dp]),
         "parent_engine_read_bins": (i32, [p, p]),
+        "parent_engine_read_bins16": (i32, [p, p]),
         "parent_engine_pair_counts": (i32, [p, i32, i32, p, p]),
         "parent_engine_mist": (i32, [p, p, dp, p, dp]),
         "parent_engine_set_entropies": (i32, [p, i32, dp, dp]),
@@ -335,6 +336,12 @@ class Engine:
     def bins(self) -> np.ndarray:
         out = np.empty((self.D, self.n_frames), dtype=np.uint8)
         self._check(lib().parent_engine_read_bins(self._h, _ptr(out)))
+        return out
+
+    def bins16(self) -> np.ndarray:
+        """The 2D bin indices as uint16 (any bin count)."""
+        out = np.empty((self.D, self.n_frames), dtype=np.uint16)
+        self._check(lib().parent_engine_read_bins16(self._h, _ptr(out)))
         return out
 
     def pair_counts(self, pairs, method: int, bins2d) -> list:

@@ -92,6 +92,8 @@ struct BinSet {
     bool s2d_valid = false;  // S2D valid
     int ent_nD = -1;         // nD of the entropies in s1d/s2d
     int method = PARENT_METHOD_PLAIN, variant = 0;  // kernel of the last pair pass
+    int idx2 = 1, idx1 = 1;  // bytes per bin index of bins2 / bins1: 1 up to 255 bins, else 2
+    int jstride = 256;       // doubles per dof row of jtab
     DevBuf flags, vmin, vmax, modfit, bins2, bins1, s1d, c2, jtab, s2d;
     TileList tiles;
     // the set whose vmin/vmax/modfit this one uses: itself (nullptr), or, in a
@@ -135,6 +137,7 @@ struct parent_engine {
     std::vector<Tuned> tuned;
     std::vector<parent_tune_entry> last_tune;
     int forced_atomic = -1;  // parent_engine_set_variant
+    int forced_slab_rows = 0;  // idem for PARENT_METHOD_SLABS: rows per slab (0: as many as the LDS holds)
 
     // the captured sweep
     hipGraph_t graph = nullptr;
@@ -184,13 +187,14 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
 Bins3 bins2d(const BinSet& s) { return Bins3{s.bins[3], s.bins[4], s.bins[5]}; }
 Bins3 bins1d(const BinSet& s) { return Bins3{s.bins[0], s.bins[1], s.bins[2]}; }
 int bmax2d(const BinSet& s) { return std::max(s.bins[3], std::max(s.bins[4], s.bins[5])); }
+int bmax1d(const BinSet& s) { return std::max(s.bins[0], std::max(s.bins[1], s.bins[2])); }
 
 BinSet& cur(parent_engine* e) { return *e->sets[e->active]; }
 
 int check_bins(parent_engine* e, const int* bins) {
     for (int k = 0; k < 6; k++) {
         if (bins[k] < 1) return fail(e, PARENT_E_INVALID, "number of bins < 1");
-        if (bins[k] > kMaxBins) return fail(e, PARENT_E_UNSUPPORTED, "more than 255 bins per dof is not supported");
+        if (bins[k] > kMaxBins) return fail(e, PARENT_E_UNSUPPORTED, "more than 4096 bins per dof is not supported");
     }
     return PARENT_OK;
 }
@@ -209,6 +213,8 @@ PairInputs pair_inputs(const Traj& t, BinSet& s) {
     in.c2 = s.c2.as<double>();
     in.logN = log((double)t.N);
     in.s2d = s.s2d.as<double>();
+    in.idx_bytes = s.idx2;
+    in.jstride = s.jstride;
     return in;
 }
 
@@ -216,6 +222,9 @@ PairInputs pair_inputs(const Traj& t, BinSet& s) {
 int alloc_set(parent_engine* e, const Traj& t, BinSet& s) {
     const int D = t.D, nD = t.nD;
     const bool need1 = s.bins[0] != s.bins[3] || s.bins[1] != s.bins[4] || s.bins[2] != s.bins[5];
+    s.idx2 = bmax2d(s) > kMaxBins8 ? 2 : 1;
+    s.idx1 = need1 ? (bmax1d(s) > kMaxBins8 ? 2 : 1) : s.idx2;
+    s.jstride = std::max(bmax2d(s), 1);
     Scratch& x = e->scratch;
     HIPCHK(e, x.emin.ensure(D * 8));
     HIPCHK(e, x.emax.ensure(D * 8));
@@ -231,12 +240,12 @@ int alloc_set(parent_engine* e, const Traj& t, BinSet& s) {
         HIPCHK(e, s.vmax.ensure(D * 8));
         HIPCHK(e, s.modfit.ensure(D * 8));
     }
-    HIPCHK(e, s.bins2.ensure((size_t)D * t.Np));
-    if (need1) HIPCHK(e, s.bins1.ensure((size_t)D * t.Np));
+    HIPCHK(e, s.bins2.ensure((size_t)D * t.Np * s.idx2));
+    if (need1) HIPCHK(e, s.bins1.ensure((size_t)D * t.Np * s.idx1));
     else s.bins1.release();
     HIPCHK(e, s.s1d.ensure(D * 8));
     HIPCHK(e, s.c2.ensure(D * 8));
-    HIPCHK(e, s.jtab.ensure((size_t)D * 256 * 8));
+    HIPCHK(e, s.jtab.ensure((size_t)D * s.jstride * 8));
     HIPCHK(e, s.s2d.ensure((size_t)std::max<long long>(n_pairs(nD), 1) * 8));
     return PARENT_OK;
 }
@@ -265,8 +274,11 @@ hipError_t enqueue_prepare(parent_engine* e, const Traj& t, BinSet& s) {
     a.vmin = const_cast<double*>(s.ranges().vmin.as<double>());
     a.vmax = const_cast<double*>(s.ranges().vmax.as<double>());
     a.modfit = const_cast<double*>(s.ranges().modfit.as<double>());
-    a.bins2 = s.bins2.as<uint8_t>();
-    a.bins1 = need1 ? s.bins1.as<uint8_t>() : nullptr;
+    a.bins2 = s.bins2.p;
+    a.bins1 = need1 ? s.bins1.p : nullptr;
+    a.idx2 = s.idx2;
+    a.idx1 = s.idx1;
+    a.jstride = s.jstride;
     a.s1d = s.s1d.as<double>();
     a.c2 = s.c2.as<double>();
     a.jtab = s.jtab.as<double>();
@@ -541,13 +553,32 @@ int tuned_choice(parent_engine* e, const Traj& tj, BinSet& s, int* method, int* 
     return PARENT_OK;
 }
 
-// The kernel a request runs on: AUTO takes the autotuner's choice, MFMA falls
-// back to the atomic kernels outside its bin range, ATOMIC falls back to the
-// plain kernel when no tile configuration fits the LDS.
+// The slab kernel serves every bin count; the others need uint8 indices and
+// a whole histogram in LDS (the plain kernel's is the largest).
+bool needs_slabs(const parent_engine* e, const BinSet& s) {
+    return s.idx2 != 1 || plain_lds_bytes(bmax2d(s)) > (size_t)e->lds_max;
+}
+
+// LDS of one slab-kernel workgroup: the whole histogram when it fits, else
+// what the device allows beside the kernel's static arrays
+size_t slab_lds(const parent_engine* e, const BinSet& s) {
+    const size_t b = (size_t)bmax2d(s), room = (size_t)e->lds_max - 256;
+    return std::min(b * b * 4, room) / 4 * 4;
+}
+
+// The kernel a request runs on: bin counts the LDS kernels cannot hold run
+// on the slab kernel whatever was asked; AUTO takes the autotuner's choice,
+// MFMA falls back to the atomic kernels outside its bin range, ATOMIC falls
+// back to the plain kernel when no tile configuration fits the LDS.
 int resolve_method(parent_engine* e, const Traj& tj, BinSet& s, int method, int* used, int* variant) {
-    if (method < PARENT_METHOD_PLAIN || method > PARENT_METHOD_AUTO) return fail(e, PARENT_E_INVALID, "unknown method");
+    if (method < PARENT_METHOD_PLAIN || method > PARENT_METHOD_SLABS) return fail(e, PARENT_E_INVALID, "unknown method");
     *used = method;
     *variant = 0;
+    if (method == PARENT_METHOD_SLABS || needs_slabs(e, s)) {
+        *used = PARENT_METHOD_SLABS;
+        *variant = e->forced_slab_rows;
+        return PARENT_OK;
+    }
     if (method == PARENT_METHOD_AUTO) return tuned_choice(e, tj, s, used, variant);
     if (method == PARENT_METHOD_MFMA) {
         if (mfma_fits(e, s)) return PARENT_OK;
@@ -566,11 +597,9 @@ int resolve_method(parent_engine* e, const Traj& tj, BinSet& s, int method, int*
 // Chooses set s's kernel and uploads its tile list (synchronises; done
 // before any capture).
 int plan_pairs(parent_engine* e, const Traj& t, BinSet& s, int method) {
-    const int bmax = bmax2d(s);
-    if ((size_t)bmax * bmax * 4 > (size_t)e->lds_max) return fail(e, PARENT_E_UNSUPPORTED, "2D histogram does not fit in LDS");
     int used = method, variant = 0;
     if (int rc = resolve_method(e, t, s, method, &used, &variant)) return rc;
-    if (used != PARENT_METHOD_PLAIN)
+    if (used != PARENT_METHOD_PLAIN && used != PARENT_METHOD_SLABS)
         if (int rc = ensure_tiles(e, t, s, used, variant)) return rc;
     s.method = used;
     s.variant = variant;
@@ -581,6 +610,7 @@ int plan_pairs(parent_engine* e, const Traj& t, BinSet& s, int method) {
 hipError_t enqueue_pairs(parent_engine* e, const Traj& t, BinSet& s) {
     const PairInputs in = with_nlogn(e, pair_inputs(t, s));
     if (s.method == PARENT_METHOD_PLAIN) return launch_pairs_plain(in, nullptr, e->stream);
+    if (s.method == PARENT_METHOD_SLABS) return launch_pairs_slabs(in, slab_lds(e, s), s.variant, nullptr, e->stream);
     const TileQueue q{s.tiles.dev.as<int>(), s.tiles.ntiles, 2, nullptr};
     return launch_tiled(e, in, bmax2d(s), s.method, s.variant, q, nullptr, nullptr, nullptr);
 }
@@ -1071,10 +1101,30 @@ int parent_engine_read_bins(parent_engine* e, uint8_t* out) {
     if (!e || !out) return PARENT_E_INVALID;
     BinSet& s = cur(e);
     if (!s.prepared) return fail(e, PARENT_E_INVALID, "parent_engine_prepare has not succeeded");
+    if (s.idx2 != 1) return fail(e, PARENT_E_UNSUPPORTED, "the indices are 16-bit (2D bins > 255): use parent_engine_read_bins16");
     HIPCHK(e, hipSetDevice(e->device));
     const Traj& t = e->traj;
     HIPCHK(e, hipMemcpy2DAsync(out, t.N, s.bins2.p, t.Np, t.N, t.D, hipMemcpyDeviceToHost, e->stream));
     HIPCHK(e, hipStreamSynchronize(e->stream));
+    return PARENT_OK;
+}
+
+int parent_engine_read_bins16(parent_engine* e, uint16_t* out) {
+    if (!e || !out) return PARENT_E_INVALID;
+    BinSet& s = cur(e);
+    if (!s.prepared) return fail(e, PARENT_E_INVALID, "parent_engine_prepare has not succeeded");
+    HIPCHK(e, hipSetDevice(e->device));
+    const Traj& t = e->traj;
+    if (s.idx2 == 2) {
+        HIPCHK(e, hipMemcpy2DAsync(out, (size_t)t.N * 2, s.bins2.p, (size_t)t.Np * 2, (size_t)t.N * 2, t.D,
+                                   hipMemcpyDeviceToHost, e->stream));
+        HIPCHK(e, hipStreamSynchronize(e->stream));
+        return PARENT_OK;
+    }
+    std::vector<uint8_t> b8((size_t)t.N * t.D);
+    HIPCHK(e, hipMemcpy2DAsync(b8.data(), t.N, s.bins2.p, t.Np, t.N, t.D, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    for (size_t k = 0; k < b8.size(); k++) out[k] = b8[k];
     return PARENT_OK;
 }
 
@@ -1100,13 +1150,14 @@ int parent_engine_pair_counts(parent_engine* e, int method, int n_pairs_req, con
     const PairInputs in = with_nlogn(e, pair_inputs(tj, s));
     int used = method, variant = 0;
     if (int rc = resolve_method(e, tj, s, method, &used, &variant)) return rc;
-    if (used == PARENT_METHOD_PLAIN) {
+    if (used == PARENT_METHOD_PLAIN || used == PARENT_METHOD_SLABS) {
         HIPCHK(e, dpairs.ensure((size_t)n_pairs_req * 8));
         HIPCHK(e, doff.ensure((size_t)n_pairs_req * 8));
         HIPCHK(e, hipMemcpyAsync(dpairs.p, pairs, (size_t)n_pairs_req * 8, hipMemcpyHostToDevice, e->stream));
         HIPCHK(e, hipMemcpyAsync(doff.p, off.data(), (size_t)n_pairs_req * 8, hipMemcpyHostToDevice, e->stream));
         CountsDump dump{n_pairs_req, dpairs.as<int>(), doff.as<long long>(), dout.as<int>()};
-        HIPCHK(e, launch_pairs_plain(in, &dump, e->stream));
+        if (used == PARENT_METHOD_SLABS) HIPCHK(e, launch_pairs_slabs(in, slab_lds(e, s), variant, &dump, e->stream));
+        else HIPCHK(e, launch_pairs_plain(in, &dump, e->stream));
     } else {
         // the tiles holding the requested pairs, each pair as (slot, offset)
         int tr, tc;
@@ -1204,7 +1255,12 @@ int parent_engine_autotune(parent_engine* e, parent_tune_entry* entries, int max
         return x.b == s.bins[3] && x.a == s.bins[4] && x.d == s.bins[5] && x.N == t.N && x.D == t.D;
     }), e->tuned.end());
     drop_graph(e);
-    if (int rc = run_autotune(e, t, s)) return rc;
+    if (needs_slabs(e, s)) {
+        // nothing to choose: only the slab kernel holds these bins
+        e->last_tune.clear();
+    } else if (int rc = run_autotune(e, t, s)) {
+        return rc;
+    }
     const int n = (int)e->last_tune.size();
     if (n_entries) *n_entries = n;
     for (int k = 0; k < n && k < max_entries && entries; k++) entries[k] = e->last_tune[k];
@@ -1213,7 +1269,13 @@ int parent_engine_autotune(parent_engine* e, parent_tune_entry* entries, int max
 
 int parent_engine_set_variant(parent_engine* e, int method, int variant) {
     if (!e) return PARENT_E_INVALID;
-    if (method != PARENT_METHOD_ATOMIC) return fail(e, PARENT_E_INVALID, "only PARENT_METHOD_ATOMIC has variants");
+    if (method == PARENT_METHOD_SLABS) {
+        if (variant < -1) return fail(e, PARENT_E_INVALID, "slab rows must be >= 0 (or -1 for the default)");
+        e->forced_slab_rows = variant < 0 ? 0 : variant;
+        drop_graph(e);
+        return PARENT_OK;
+    }
+    if (method != PARENT_METHOD_ATOMIC) return fail(e, PARENT_E_INVALID, "only PARENT_METHOD_ATOMIC and PARENT_METHOD_SLABS have variants");
     if (variant < -1 || variant >= atomic_config_count()) return fail(e, PARENT_E_INVALID, "variant out of range");
     e->forced_atomic = variant;
     return PARENT_OK;
@@ -1222,7 +1284,8 @@ int parent_engine_set_variant(parent_engine* e, int method, int variant) {
 int parent_engine_variant_count(parent_engine* e, int method, int* count) {
     if (!e || !count) return PARENT_E_INVALID;
     *count = method == PARENT_METHOD_ATOMIC ? atomic_config_count()
-                                            : (method == PARENT_METHOD_PLAIN || method == PARENT_METHOD_MFMA ? 1 : 0);
+                                            : (method == PARENT_METHOD_PLAIN || method == PARENT_METHOD_MFMA ||
+                                               method == PARENT_METHOD_SLABS ? 1 : 0);
     return PARENT_OK;
 }
 
@@ -1248,7 +1311,7 @@ int parent_batch_create(int device, const int bins[6], parent_batch** out) {
     *out = nullptr;
     for (int k = 0; k < 6; k++)
         if (bins[k] < 1 || bins[k] > kMaxBins) {
-            set_last_error(bins[k] < 1 ? "number of bins < 1" : "more than 255 bins per dof is not supported");
+            set_last_error(bins[k] < 1 ? "number of bins < 1" : "more than 4096 bins per dof is not supported");
             return bins[k] < 1 ? PARENT_E_INVALID : PARENT_E_UNSUPPORTED;
         }
     parent_engine* e = nullptr;
@@ -1334,7 +1397,32 @@ int parent_batch_run(parent_batch* b, int method) {
     BinSet& sb = b->items[big]->set;
     const Traj& tb = b->items[big]->traj;
     int used = method, variant = 0;
-    if (method < PARENT_METHOD_PLAIN || method > PARENT_METHOD_AUTO) return bfail(b, PARENT_E_INVALID, "unknown method");
+    if (method < PARENT_METHOD_PLAIN || method > PARENT_METHOD_SLABS) return bfail(b, PARENT_E_INVALID, "unknown method");
+    if (method == PARENT_METHOD_SLABS || needs_slabs(e, sb)) {
+        // bins the LDS kernels cannot hold: the slab kernel, one launch per
+        // trajectory (it has no tile queue)
+        HIPCHK(e, hipEventRecord(e->ev[4], e->stream));
+        double upd = 0;
+        for (auto& it : b->items) {
+            const PairInputs in = with_nlogn(e, pair_inputs(it->traj, it->set));
+            HIPCHK(e, launch_pairs_slabs(in, slab_lds(e, it->set), e->forced_slab_rows, nullptr, e->stream));
+            upd += (double)n_pairs(it->traj.nD) * it->traj.N;
+        }
+        HIPCHK(e, hipEventRecord(e->ev[5], e->stream));
+        HIPCHK(e, hipStreamSynchronize(e->stream));
+        for (auto& it : b->items) {
+            it->set.s2d_valid = true;
+            it->set.method = PARENT_METHOD_SLABS;
+            it->set.variant = e->forced_slab_rows;
+        }
+        b->method = PARENT_METHOD_SLABS;
+        b->variant = e->forced_slab_rows;
+        e->stats.ms_pairs = elapsed(e->ev[4], e->ev[5]);
+        e->stats.pair_updates = upd;
+        e->stats.method = PARENT_METHOD_SLABS;
+        e->stats.kernel_variant = e->forced_slab_rows;
+        return PARENT_OK;
+    }
     if (method == PARENT_METHOD_AUTO) {
         if (b->tuned_items != n) {
             if (int rc = batch_autotune(e, b, &b->tuned_method, &b->tuned_variant)) return rc;

@@ -20,10 +20,12 @@ struct PrepareArgs {
     unsigned* mfhist;                 // [nD][100] scratch
     int* flags;
     double *vmin, *vmax, *modfit;     // [D]
-    uint8_t* bins2;                   // [D][Np]
-    uint8_t* bins1;                   // [D][Np] or nullptr when 1D bins == 2D bins for all types
+    void* bins2;                      // [D][Np] indices of idx2 bytes (uint8 or uint16)
+    void* bins1;                      // [D][Np] of idx1 bytes, or nullptr when 1D bins == 2D bins for all types
+    int idx2, idx1;                   // index bytes: 1 while the counts are <= 255, else 2
     double *s1d, *c2;                 // [D]
-    double* jtab;                     // [D][256]
+    double* jtab;                     // [D][jstride]: 2D-bin Jacobians
+    int jstride;                      // >= every 2D bin count
     double* nlogn;                    // [N+1]: n ln n - 1/2, and 0 at n = 0
 };
 hipError_t launch_prepare(const PrepareArgs& a, hipStream_t s);
@@ -36,12 +38,14 @@ hipError_t launch_bat_records(const void* records, long long rec_bytes, int n_fr
 
 // Common inputs of every pair kernel.
 struct PairInputs {
-    const uint8_t* bins;  // dof-major [D][Np]
+    const uint8_t* bins;  // dof-major [D][Np] indices of idx_bytes each (uint8 for all kernels but the slab one)
     int nD, D, N, Np;
     Bins3 B;              // 2D bins per type
     const double *vmin, *vmax, *jtab, *c2, *nlogn;
     double logN;
     double* s2d;          // [D(D-1)/2] .par order
+    int idx_bytes;        // 1 or 2
+    int jstride;          // doubles per dof row of jtab
 };
 
 // A work list of pair tiles.  One trajectory: tiles[t] = (r0, c0), stride 2,
@@ -65,6 +69,14 @@ struct CountsDump {
 
 // plain: one pair per workgroup
 hipError_t launch_pairs_plain(const PairInputs& in, const CountsDump* dump, hipStream_t s);
+// LDS bytes the plain kernel needs for 2D bin counts up to bmax (histogram and reduction arrays)
+size_t plain_lds_bytes(int bmax);
+
+// slabs: one pair per workgroup, the histogram in row slabs that fit
+// lds_bytes of LDS (every bin count up to kMaxBins, uint8 or uint16
+// indices); max_rows > 0 caps the rows of a slab
+hipError_t launch_pairs_slabs(const PairInputs& in, size_t lds_bytes, int max_rows, const CountsDump* dump,
+                              hipStream_t s);
 
 // tiled LDS-atomic kernel
 struct AtomicConfig {

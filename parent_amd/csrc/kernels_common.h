@@ -7,8 +7,12 @@
 namespace parent {
 
 constexpr int kModfitBins = 100;   // PARENT.cpp MODFITNBINS
-constexpr int kMaxBins = 255;      // bin indices are stored as uint8; 0xFF marks padding frames
+// Bin indices are stored as uint8 while every count of a matrix is <= 255
+// (0xFF marks padding frames), as uint16 up to kMaxBins (0xFFFF marks them).
+constexpr int kMaxBins8 = 255;
+constexpr int kMaxBins = 4096;     // bins per dof the library accepts
 constexpr uint8_t kPadBin = 0xFF;
+constexpr uint16_t kPadBin16 = 0xFFFF;
 constexpr int kFrameAlign = 4096;  // frame stride of the dof-major bin-index rows
 
 // error bits raised by kernels (engine flags word)

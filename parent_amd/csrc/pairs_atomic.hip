@@ -140,8 +140,8 @@ __global__ __launch_bounds__(256) void k_pairs_plain(PairInputs in, long long p0
     }
     const double b1 = (in.vmax[gi] - in.vmin[gi]) / Bi;
     const double b2 = (in.vmax[gj] - in.vmin[gj]) / Bj;
-    const double* J1 = in.jtab + (size_t)gi * 256;
-    const double* J2 = in.jtab + (size_t)gj * 256;
+    const double* J1 = in.jtab + (size_t)gi * in.jstride;
+    const double* J2 = in.jtab + (size_t)gj * in.jstride;
     const int numFrames = N;
     double sum = 0.0;
     int occ = 0;
@@ -170,6 +170,9 @@ __global__ __launch_bounds__(256) void k_pairs_plain(PairInputs in, long long p0
     if (threadIdx.x == 0)
         in.s2d[par_offset(in.nD, gi, gj)] = -rs[0] + (ro[0] - 1.0) / (2.0 * numFrames);
 }
+
+// dynamic histogram + the static reduction arrays (rs, ro)
+size_t plain_lds_bytes(int bmax) { return (size_t)bmax * bmax * sizeof(unsigned) + 256 * (sizeof(double) + sizeof(int)); }
 
 hipError_t launch_pairs_plain(const PairInputs& in, const CountsDump* dump, hipStream_t s) {
     const int bmax = max(in.B.b, max(in.B.a, in.B.d));

@@ -6,7 +6,7 @@
 // and "PROGRAM FINISHED SUCCESSFULLY.", PARENT.cpp:1501-1509) and exit codes
 // (1 on any error, PARENT.cpp:55 RC_ABORT) as the reference, so run.sh:40 and
 // the .par consumers work unchanged.  Optional environment knob:
-//   PARENT_HIP_METHOD = auto | plain | atomic | mfma   (pair kernel, default auto)
+//   PARENT_HIP_METHOD = auto | plain | atomic | mfma | slabs   (pair kernel, default auto)
 //   PARENT_HIP_DEVICE = <n>                            (HIP device, default 0)
 #include "../../include/parent_hip.h"
 
@@ -48,6 +48,7 @@ int method_from_env() {
     if (!strcmp(m, "plain")) return PARENT_METHOD_PLAIN;
     if (!strcmp(m, "atomic")) return PARENT_METHOD_ATOMIC;
     if (!strcmp(m, "mfma")) return PARENT_METHOD_MFMA;
+    if (!strcmp(m, "slabs")) return PARENT_METHOD_SLABS;
     return PARENT_METHOD_AUTO;
 }
 

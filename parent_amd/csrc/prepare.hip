@@ -271,18 +271,29 @@ __global__ void k_finalize_dihedral(int D, int nD, double twopi,
 // Pass 5: bin indices int((x-min)/((max-min)/B)) for every (frame, dof)
 // (PARENT.cpp:654-656 and the 2D fills, 802-804 ...), written transposed to
 // the dof-major rows through an LDS tile of 64 dofs x 256 frames so both the
-// frame-major reads and the dof-major writes are coalesced.
-template <class T, bool NEED1>
+// frame-major reads and the dof-major writes are coalesced.  I2 / I1: the
+// index types (uint8_t, or uint16_t for bin counts above 255) of the 2D and
+// 1D matrices; the tiles are dynamic LDS of 64 x kPrebinRow indices each.
+constexpr int kPrebinRow = 260;  // 256 frames + 4: rows start in different LDS banks
+
+template <class I2, class I1, bool NEED1>
+constexpr size_t prebin_lds() {
+    return (size_t)64 * kPrebinRow * (sizeof(I2) + (NEED1 ? sizeof(I1) : 0));
+}
+
+template <class T, bool NEED1, class I2, class I1>
 __global__ __launch_bounds__(256) void k_prebin(const T* __restrict__ v, int N, int Np, int D, int nD,
                                                 const double* __restrict__ vmin,
                                                 const double* __restrict__ vmax,
                                                 const double* __restrict__ modfit, double twopi,
                                                 Bins3 B2, Bins3 B1, int nbx, int nby,
-                                                uint8_t* __restrict__ bins2, uint8_t* __restrict__ bins1,
+                                                void* __restrict__ bins2, void* __restrict__ bins1,
                                                 int* flags) {
-    constexpr int kRow = 260;  // 256 frames + 4: rows start in different LDS banks
-    __shared__ uint8_t tile2[64 * kRow];
-    __shared__ uint8_t tile1[NEED1 ? 64 * kRow : 4];
+    extern __shared__ __attribute__((aligned(16))) unsigned char prebin_smem[];
+    I2* tile2 = reinterpret_cast<I2*>(prebin_smem);
+    I1* tile1 = reinterpret_cast<I1*>(prebin_smem + (size_t)64 * kPrebinRow * sizeof(I2));
+    constexpr I2 pad2 = (I2)~(I2)0;
+    constexpr I1 pad1 = (I1)~(I1)0;
     int bx, by;
     if (!xcd_block(nbx, nby, &bx, &by)) return;
     const int dl = threadIdx.x & 63, fl = threadIdx.x >> 6;
@@ -308,8 +319,8 @@ __global__ __launch_bounds__(256) void k_prebin(const T* __restrict__ v, int N, 
     const int fend = g < D ? min(N, f0 + 256) : f0;
     for (int r = fl; r < 256; r += 4)
         if (f0 + r >= fend) {
-            tile2[dl * kRow + r] = kPadBin;
-            if constexpr (NEED1) tile1[dl * kRow + r] = kPadBin;
+            tile2[dl * kPrebinRow + r] = pad2;
+            if constexpr (NEED1) tile1[dl * kPrebinRow + r] = pad1;
         }
     for_frames<16>(v, D, g, f0 + fl, fend, [&](int f, double x) {
         const int r = f - f0;
@@ -319,7 +330,7 @@ __global__ __launch_bounds__(256) void k_prebin(const T* __restrict__ v, int N, 
             bad = true;
             k = k < 0 ? 0 : nb2 - 1;
         }
-        tile2[dl * kRow + r] = (uint8_t)k;
+        tile2[dl * kPrebinRow + r] = (I2)k;
         if constexpr (NEED1) {
             int k1 = k;
             if (sep1) {
@@ -329,20 +340,46 @@ __global__ __launch_bounds__(256) void k_prebin(const T* __restrict__ v, int N, 
                     k1 = k1 < 0 ? 0 : nb1 - 1;
                 }
             }
-            tile1[dl * kRow + r] = (uint8_t)k1;
+            tile1[dl * kPrebinRow + r] = (I1)k1;
         }
     });
     if (bad) atomicOr(flags, kFlagRange);
     __syncthreads();
-    for (int w = threadIdx.x; w < 64 * 64; w += 256) {
-        const int dof = w >> 6, wr = w & 63;
+    // rows of 256 frames leave as 32-bit words: 64 (uint8) or 128 (uint16) a row
+    constexpr int w2 = 256 * (int)sizeof(I2) / 4, w1 = 256 * (int)sizeof(I1) / 4;
+    for (int w = threadIdx.x; w < 64 * w2; w += 256) {
+        const int dof = w / w2, wr = w % w2;
         const int gg = d0 + dof;
         if (gg >= D) continue;
-        const uint32_t word2 = *reinterpret_cast<const uint32_t*>(&tile2[dof * kRow + 4 * wr]);
-        *reinterpret_cast<uint32_t*>(&bins2[(size_t)gg * Np + f0 + 4 * wr]) = word2;
-        if constexpr (NEED1) {
-            const uint32_t word1 = *reinterpret_cast<const uint32_t*>(&tile1[dof * kRow + 4 * wr]);
-            *reinterpret_cast<uint32_t*>(&bins1[(size_t)gg * Np + f0 + 4 * wr]) = word1;
+        const uint32_t word = *reinterpret_cast<const uint32_t*>(tile2 + dof * kPrebinRow + wr * (4 / sizeof(I2)));
+        *reinterpret_cast<uint32_t*>(static_cast<I2*>(bins2) + (size_t)gg * Np + f0 + wr * (4 / sizeof(I2))) = word;
+    }
+    if constexpr (NEED1) {
+        for (int w = threadIdx.x; w < 64 * w1; w += 256) {
+            const int dof = w / w1, wr = w % w1;
+            const int gg = d0 + dof;
+            if (gg >= D) continue;
+            const uint32_t word = *reinterpret_cast<const uint32_t*>(tile1 + dof * kPrebinRow + wr * (4 / sizeof(I1)));
+            *reinterpret_cast<uint32_t*>(static_cast<I1*>(bins1) + (size_t)gg * Np + f0 + wr * (4 / sizeof(I1))) = word;
+        }
+    }
+}
+
+// Histogram of one dof's index row (Np indices of type I, padding skipped)
+// into the LDS counters h.
+template <class I>
+__device__ __forceinline__ void row_histogram(const I* __restrict__ row, int Np, unsigned* h) {
+    constexpr int per = 16 / (int)sizeof(I);  // indices per 16-byte load
+    constexpr unsigned pad = (unsigned)(I)~(I)0;
+    const uint4* r = reinterpret_cast<const uint4*>(row);
+    for (int i = threadIdx.x; i < Np / per; i += 256) {
+        const uint4 w = r[i];
+        const unsigned ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+        for (int q = 0; q < per; q++) {
+            const unsigned b = sizeof(I) == 1 ? (ws[q >> 2] >> (8 * (q & 3))) & 0xFFu
+                                              : (ws[q >> 1] >> (16 * (q & 1))) & 0xFFFFu;
+            if (b != pad) atomicAdd(&h[b], 1u);
         }
     }
 }
@@ -356,107 +393,98 @@ __global__ __launch_bounds__(256) void k_prebin(const T* __restrict__ v, int N, 
 // With these, S_ij = ln N + c2[i] + c2[j] - (1/N) sum n ln n + (occ-1)/(2N),
 // which is the reference epilogue regrouped (the marginal of a 2D histogram
 // is the dof's 1D histogram at the 2D bin count).
-__global__ __launch_bounds__(256) void k_dof_stats(const uint8_t* __restrict__ bins2,
-                                                   const uint8_t* __restrict__ bins1, int N, int Np,
-                                                   int nD, Bins3 B1, Bins3 B2,
-                                                   const double* __restrict__ vmin,
-                                                   const double* __restrict__ vmax, double* s1d,
-                                                   double* c2, double* jtab) {
-    __shared__ unsigned h2[256], h1[256];
+//
+// One workgroup per dof.  The bin-centre recurrences run in one thread (each
+// centre is the previous one plus the bin size, as in the reference); the
+// per-bin terms (divisions, log, sin) then run one bin per thread, and one
+// thread adds them in bin order, so every sum has the reference's order and
+// bits.  Dynamic LDS (dof_stats_lds): counters h2 [nb2] and h1 [nh1] (the
+// separate 1D histogram, nh1 = 0 without a 1D matrix), then doubles t2
+// [nb2] and t1 [nb1]: the bin centres, overwritten by the terms.
+template <class I2, class I1>
+__global__ __launch_bounds__(256) void k_dof_stats(const void* __restrict__ idx2, const void* __restrict__ idx1,
+                                                   int N, int Np, int nD, Bins3 B1, Bins3 B2, int nb2, int nh1,
+                                                   int nb1, const double* __restrict__ vmin,
+                                                   const double* __restrict__ vmax, double* s1d, double* c2,
+                                                   double* jtab, int jstride) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char stats_smem[];
+    unsigned* h2 = reinterpret_cast<unsigned*>(stats_smem);
+    unsigned* h1 = h2 + nb2;
+    double* t2 = reinterpret_cast<double*>(stats_smem + (size_t)((nb2 + nh1 + 1) / 2) * 8);
+    double* t1 = t2 + nb2;
     const int g = blockIdx.x;
     const int t = dof_type(nD, g);
-    const bool sep1 = bins1 != nullptr && B1.of(t) != B2.of(t);
-    h2[threadIdx.x] = 0;
-    h1[threadIdx.x] = 0;
+    const bool sep1 = idx1 != nullptr && B1.of(t) != B2.of(t);
+    for (int i = threadIdx.x; i < nb2 + nh1; i += 256) h2[i] = 0;
     __syncthreads();
-    const uint4* r2 = reinterpret_cast<const uint4*>(bins2 + (size_t)g * Np);
-    const uint4* r1 = reinterpret_cast<const uint4*>((sep1 ? bins1 : bins2) + (size_t)g * Np);
-    for (int i = threadIdx.x; i < Np / 16; i += 256) {
-        const uint4 w = r2[i];
-        const unsigned ws[4] = {w.x, w.y, w.z, w.w};
-#pragma unroll
-        for (int q = 0; q < 16; q++) {
-            const unsigned b = (ws[q >> 2] >> (8 * (q & 3))) & 0xFFu;
-            if (b != kPadBin) atomicAdd(&h2[b], 1u);
+    row_histogram(static_cast<const I2*>(idx2) + (size_t)g * Np, Np, h2);
+    if (sep1) row_histogram(static_cast<const I1*>(idx1) + (size_t)g * Np, Np, h1);
+    const unsigned* histo = sep1 ? h1 : h2;
+    const int numFrames = N;
+    const int bins = B1.of(t), bins2 = B2.of(t);
+    const double binsize = (vmax[g] - vmin[g]) / (bins);
+    const double binsize2 = (vmax[g] - vmin[g]) / bins2;
+    // bin centres (PARENT.cpp:660-667, 704-712; 2D: 808-819)
+    if (threadIdx.x == 0 && t != 2) {
+        double c = vmin[g] + (binsize / 2.0);
+        for (int k = 0; k < bins; k++) {
+            t1[k] = c;
+            c += binsize;
         }
-        if (sep1) {
-            const uint4 u = r1[i];
-            const unsigned us[4] = {u.x, u.y, u.z, u.w};
-#pragma unroll
-            for (int q = 0; q < 16; q++) {
-                const unsigned b = (us[q >> 2] >> (8 * (q & 3))) & 0xFFu;
-                if (b != kPadBin) atomicAdd(&h1[b], 1u);
-            }
+        c = vmin[g] + binsize2 / 2.0;
+        for (int k = 0; k < bins2; k++) {
+            t2[k] = c;
+            c = c + binsize2;
         }
+    }
+    __syncthreads();
+    // per-bin terms: t1[k] = J p ln p (the 1D sum's term when p > 0), t2[k] =
+    // h2[k] ln J; the Jacobian table
+    double* jt = jtab + (size_t)g * jstride;
+    for (int k = threadIdx.x; k < bins; k += 256) {
+        double term = 0.0;
+        if (t == 0) {
+            const double blen = t1[k];
+            const double probDens = histo[k] / (numFrames * binsize * blen * blen);
+            if (probDens > 0) term = blen * blen * probDens * log(probDens);
+        } else if (t == 1) {
+            const double theta = t1[k];
+            const double probDens = histo[k] / (numFrames * binsize * sin(theta));
+            if (probDens > 0) term = sin(theta) * probDens * log(probDens);
+        } else {
+            const double probDens = histo[k] / (numFrames * binsize);
+            if (probDens > 0) term = probDens * log(probDens);
+        }
+        t1[k] = term;
+    }
+    for (int k = threadIdx.x; k < bins2; k += 256) {
+        double J = 1.0;
+        if (t == 0) J = t2[k] * t2[k];
+        else if (t == 1) J = sin(t2[k]);
+        jt[k] = J;
+        t2[k] = (t != 2 && h2[k]) ? (double)h2[k] * log(J) : 0.0;
     }
     __syncthreads();
     if (threadIdx.x != 0) return;
-    const unsigned* histo = sep1 ? h1 : h2;
-    const int numFrames = N;
-    {
-        const int bins = B1.of(t);
-        const double binsize = (vmax[g] - vmin[g]) / (bins);
-        int occupbins = 0;
-        double plnpsum = 0, probDens;
-        if (t == 0) {
-            double blen = vmin[g] + (binsize / 2.0);
-            for (int k = 0; k < bins; k++) {
-                probDens = histo[k] / (numFrames * binsize * blen * blen);
-                if (probDens > 0) {
-                    plnpsum = plnpsum + blen * blen * probDens * log(probDens);
-                    occupbins = occupbins + 1;
-                }
-                blen += binsize;
-            }
-        } else if (t == 1) {
-            double theta = vmin[g] + (binsize / 2.0);
-            for (int k = 0; k < bins; k++) {
-                probDens = histo[k] / (numFrames * binsize * sin(theta));
-                if (probDens > 0) {
-                    plnpsum = plnpsum + sin(theta) * probDens * log(probDens);
-                    occupbins = occupbins + 1;
-                }
-                theta += binsize;
-            }
-        } else {
-            for (int k = 0; k < bins; k++) {
-                probDens = histo[k] / (numFrames * binsize);
-                if (probDens > 0) {
-                    plnpsum = plnpsum + probDens * log(probDens);
-                    occupbins = occupbins + 1;
-                }
-            }
+    // the sums in bin order (PARENT.cpp:650-671 ...): a bin counts when its
+    // density is > 0, i.e. when its count is
+    int occupbins = 0;
+    double plnpsum = 0;
+    for (int k = 0; k < bins; k++)
+        if (histo[k]) {
+            plnpsum = plnpsum + t1[k];
+            occupbins = occupbins + 1;
         }
-        plnpsum = -plnpsum * binsize;
-        s1d[g] = plnpsum + (occupbins - 1.0) / (2.0 * numFrames);
-    }
-    {
-        const int bins = B2.of(t);
-        const double binsize = (vmax[g] - vmin[g]) / bins;
-        double acc = 0.0;
-        double* jt = jtab + (size_t)g * 256;
-        if (t == 0) {
-            double blen = vmin[g] + binsize / 2.0;
-            for (int k = 0; k < bins; k++) {
-                const double J = blen * blen;
-                jt[k] = J;
-                if (h2[k]) acc += (double)h2[k] * log(J);
-                blen = blen + binsize;
-            }
-        } else if (t == 1) {
-            double theta = vmin[g] + binsize / 2.0;
-            for (int k = 0; k < bins; k++) {
-                const double J = sin(theta);
-                jt[k] = J;
-                if (h2[k]) acc += (double)h2[k] * log(J);
-                theta = theta + binsize;
-            }
-        } else {
-            for (int k = 0; k < bins; k++) jt[k] = 1.0;
-        }
-        c2[g] = log(binsize) + acc / numFrames;
-    }
+    plnpsum = -plnpsum * binsize;
+    s1d[g] = plnpsum + (occupbins - 1.0) / (2.0 * numFrames);
+    double acc = 0.0;
+    if (t != 2)
+        for (int k = 0; k < bins2; k++)
+            if (h2[k]) acc += t2[k];
+    c2[g] = log(binsize2) + acc / numFrames;
 }
+
+size_t dof_stats_lds(int nb2, int nh1, int nb1) { return (size_t)((nb2 + nh1 + 1) / 2) * 8 + (size_t)(nb2 + nb1) * 8; }
 
 // n ln n - 1/2 for n = 1..N and 0 for n = 0, in FP64.  The 2D epilogue sums
 // table entries over the cells: the occupancy term (occ - 1) / (2N) of
@@ -471,6 +499,34 @@ __global__ void k_nlogn(int N, double* tab) {
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+
+template <class T, bool NEED1, class I2, class I1>
+hipError_t prebin_launch(const PrepareArgs& a, unsigned g5, int nbx5, int nby5, double twopi, hipStream_t s) {
+    constexpr size_t lds = prebin_lds<I2, I1, NEED1>();
+    if (lds > 65536) {
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_prebin<T, NEED1, I2, I1>),
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    k_prebin<T, NEED1, I2, I1><<<g5, 256, lds, s>>>(static_cast<const T*>(a.values), a.N, a.Np, a.D, a.nD, a.vmin,
+                                                     a.vmax, a.modfit, twopi, a.B2, a.B1, nbx5, nby5, a.bins2,
+                                                     a.bins1, a.flags);
+    return hipSuccess;
+}
+
+// k_prebin for the matrices' index types
+template <class T>
+hipError_t prebin_index_types(const PrepareArgs& a, unsigned g5, int nbx5, int nby5, double twopi, hipStream_t s) {
+    if (!a.bins1) {
+        return a.idx2 == 1 ? prebin_launch<T, false, uint8_t, uint8_t>(a, g5, nbx5, nby5, twopi, s)
+                           : prebin_launch<T, false, uint16_t, uint16_t>(a, g5, nbx5, nby5, twopi, s);
+    }
+    if (a.idx2 == 1)
+        return a.idx1 == 1 ? prebin_launch<T, true, uint8_t, uint8_t>(a, g5, nbx5, nby5, twopi, s)
+                           : prebin_launch<T, true, uint8_t, uint16_t>(a, g5, nbx5, nby5, twopi, s);
+    return a.idx1 == 1 ? prebin_launch<T, true, uint16_t, uint8_t>(a, g5, nbx5, nby5, twopi, s)
+                       : prebin_launch<T, true, uint16_t, uint16_t>(a, g5, nbx5, nby5, twopi, s);
+}
 
 hipError_t launch_prepare(const PrepareArgs& a, hipStream_t s) {
     const int D = a.D, N = a.N, nD = a.nD;
@@ -503,16 +559,24 @@ hipError_t launch_prepare(const PrepareArgs& a, hipStream_t s) {
     if (a.stages & kPrepareBins) {
         const int nbx5 = (D + 63) / 64, nby5 = a.Np / 256;
         const unsigned g5 = xcd_grid(nbx5, nby5);
-        const bool need1 = a.bins1 != nullptr;
-        if (a.double_prec && need1)
-            k_prebin<double, true><<<g5, 256, 0, s>>>((const double*)a.values, N, a.Np, D, nD, a.vmin, a.vmax, a.modfit, twopi, a.B2, a.B1, nbx5, nby5, a.bins2, a.bins1, a.flags);
-        else if (a.double_prec)
-            k_prebin<double, false><<<g5, 256, 0, s>>>((const double*)a.values, N, a.Np, D, nD, a.vmin, a.vmax, a.modfit, twopi, a.B2, a.B1, nbx5, nby5, a.bins2, a.bins1, a.flags);
-        else if (need1)
-            k_prebin<float, true><<<g5, 256, 0, s>>>((const float*)a.values, N, a.Np, D, nD, a.vmin, a.vmax, a.modfit, twopi, a.B2, a.B1, nbx5, nby5, a.bins2, a.bins1, a.flags);
-        else
-            k_prebin<float, false><<<g5, 256, 0, s>>>((const float*)a.values, N, a.Np, D, nD, a.vmin, a.vmax, a.modfit, twopi, a.B2, a.B1, nbx5, nby5, a.bins2, a.bins1, a.flags);
-        k_dof_stats<<<D, 256, 0, s>>>(a.bins2, a.bins1, N, a.Np, nD, a.B1, a.B2, a.vmin, a.vmax, a.s1d, a.c2, a.jtab);
+        hipError_t e = a.double_prec ? prebin_index_types<double>(a, g5, nbx5, nby5, twopi, s)
+                                     : prebin_index_types<float>(a, g5, nbx5, nby5, twopi, s);
+        if (e != hipSuccess) return e;
+        const int nb2 = max(a.B2.b, max(a.B2.a, a.B2.d)), nb1 = max(a.B1.b, max(a.B1.a, a.B1.d));
+        const int nh1 = a.bins1 ? nb1 : 0;
+        const size_t lds = dof_stats_lds(nb2, nh1, nb1);
+        const void* fn = a.idx2 == 1 ? (a.idx1 == 1 ? (const void*)&k_dof_stats<uint8_t, uint8_t> : (const void*)&k_dof_stats<uint8_t, uint16_t>)
+                                     : (a.idx1 == 1 ? (const void*)&k_dof_stats<uint16_t, uint8_t> : (const void*)&k_dof_stats<uint16_t, uint16_t>);
+        if (lds > 65536 && (e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)) != hipSuccess)
+            return e;
+#define PARENT_STATS(I2, I1)                                                                                   \
+    k_dof_stats<I2, I1><<<D, 256, lds, s>>>(a.bins2, a.bins1, N, a.Np, nD, a.B1, a.B2, nb2, nh1, nb1, a.vmin, \
+                                            a.vmax, a.s1d, a.c2, a.jtab, a.jstride)
+        if (a.idx2 == 1 && a.idx1 == 1) PARENT_STATS(uint8_t, uint8_t);
+        else if (a.idx2 == 1) PARENT_STATS(uint8_t, uint16_t);
+        else if (a.idx1 == 1) PARENT_STATS(uint16_t, uint8_t);
+        else PARENT_STATS(uint16_t, uint16_t);
+#undef PARENT_STATS
     }
     if (a.stages & kPrepareNlogn) k_nlogn<<<(N + 256) / 256, 256, 0, s>>>(N, a.nlogn);
     return hipGetLastError();

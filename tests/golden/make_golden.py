@@ -10,6 +10,9 @@ sources by `make -C oracle ref` (oracle/_ref/):
   chain12_b50_MIST.txt        get_PAR_MIST.x chain12_b50.par
   chain12_mixed.par           PARENT.x ... 7 13 29 5 40 17
   chain12_mixed_MIST.txt      get_PAR_MIST.x chain12_mixed.par
+  chain12_wide.par            PARENT.x ... 210 256 300 500 256 210 (bin counts past uint8 indices
+                              and past a 2D histogram in LDS)
+  chain12_wide_MIST.txt       get_PAR_MIST.x chain12_wide.par
   ubq_sample_MIE_tail.txt     test_system/sample_output/sample_output_MIE.txt (reference data)
   ubq_sample_MIST_tail.txt    test_system/sample_output/sample_output_MIST.txt (reference data)
   ubq/UBQ_UBM2.top, .xtc      test_system/UBQ_UBM2.top / .xtc (reference data: the inputs of its
@@ -70,7 +73,7 @@ def main():
 
     bat = os.path.join(HERE, "chain12_1000.bat")
     parent_amd.synth_write_bat(bat, 12, 1000, 3)
-    for tag, bins in (("b50", [50] * 6), ("mixed", [7, 13, 29, 5, 40, 17])):
+    for tag, bins in (("b50", [50] * 6), ("mixed", [7, 13, 29, 5, 40, 17]), ("wide", [210, 256, 300, 500, 256, 210])):
         par = os.path.join(HERE, f"chain12_{tag}.par")
         run([os.path.join(REF, "PARENT.x"), bat, par] + [str(b) for b in bins], cwd=HERE)
         with tempfile.TemporaryDirectory() as td:

@@ -33,7 +33,8 @@ def _fmt(x):
     return "%.12g" % x
 
 
-@pytest.mark.parametrize("tag,bins", [("b50", [50] * 6), ("mixed", [7, 13, 29, 5, 40, 17])])
+@pytest.mark.parametrize("tag,bins", [("b50", [50] * 6), ("mixed", [7, 13, 29, 5, 40, 17]),
+                                      ("wide", [210, 256, 300, 500, 256, 210])])
 def test_oracle_executable_matches_reference_par(tmp_path, tag, bins):
     out = str(tmp_path / "o.par")
     r = subprocess.run([O.ORACLE_EXE, BAT, out] + [str(b) for b in bins], capture_output=True, text=True,
@@ -43,7 +44,7 @@ def test_oracle_executable_matches_reference_par(tmp_path, tag, bins):
     assert open(out, "rb").read() == open(os.path.join(GOLD, f"chain12_{tag}.par"), "rb").read()
 
 
-@pytest.mark.parametrize("tag", ["b50", "mixed"])
+@pytest.mark.parametrize("tag", ["b50", "mixed", "wide"])
 def test_oracle_mist_matches_reference_text(tag):
     import parent_amd
     nD = 9

@@ -399,8 +399,11 @@ def test_bad_inputs_raise(engine):
     assert ei.value.code == 5
     engine.set_trajectory(vals)
     with pytest.raises(pa.ParentError) as ei:
-        engine.prepare([300] * 6)
+        engine.prepare([4097] + [20] * 5)  # above the library's 4,096 bins per dof
     assert ei.value.code == 7
+    with pytest.raises(pa.ParentError) as ei:
+        engine.prepare([0] + [20] * 5)  # PARENT.cpp:211-234: bins < 1
+    assert ei.value.code == 1
 
 
 @pytest.mark.parametrize("bins,expect_method", [(5, 2), (50, 1)])
