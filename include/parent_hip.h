@@ -328,8 +328,17 @@ const char* parent_batch_error(const parent_batch* b);
 int parent_batch_add(parent_batch* b, int n_dihedrals, int n_frames, int double_prec,
                      const void* values, int on_device, int* index);
 
+/* The same with this trajectory's own bins (as for parent_engine_prepare):
+ * trajectories of different bin counts share the one launch of
+ * parent_batch_run, whose kernel is chosen to hold the largest counts of the
+ * batch (each tile reads its trajectory's counts).  The reference runs
+ * PARENT.x once per trajectory and bin setting (run.sh:40). */
+int parent_batch_add_binned(parent_batch* b, const int bins[6], int n_dihedrals, int n_frames, int double_prec,
+                            const void* values, int on_device, int* index);
+
 /* All pairs of all added trajectories, one kernel launch (method as for
- * parent_engine_pairs; the tile shape is chosen on the largest trajectory;
+ * parent_engine_pairs; the tile shape is chosen for the batch's largest bin
+ * counts;
  * PARENT_METHOD_PLAIN runs the tiled atomic kernel, as the plain kernel has
  * no tile queue).  stats.ms_pairs is that launch. */
 int parent_batch_run(parent_batch* b, int method);

@@ -72,6 +72,7 @@ def lib() -> C.CDLL:
         "parent_batch_destroy": (None, [p]),
         "parent_batch_error": (C.c_char_p, [p]),
         "parent_batch_add": (i32, [p, i32, i32, i32, p, i32, C.POINTER(i32)]),
+        "parent_batch_add_binned": (i32, [p, p, i32, i32, i32, p, i32, C.POINTER(i32)]),
         "parent_batch_run": (i32, [p, i32]),
         "parent_batch_read": (i32, [p, i32, dp, dp]),
         "parent_batch_mist": (i32, [p, i32, p, dp, p, dp]),
@@ -443,15 +444,23 @@ class Batch:
         if rc != 0:
             raise ParentError(rc, lib().parent_batch_error(self._h).decode())
 
-    def add(self, values) -> int:
-        """Adds a frame-major [N][D] float32/float64 host array; returns its index."""
+    def add(self, values, bins=None) -> int:
+        """Adds a frame-major [N][D] float32/float64 host array, with its own
+        bins[6] or the batch's; returns its index."""
         v = np.ascontiguousarray(values)
         if v.dtype not in (np.float32, np.float64) or v.ndim != 2 or (v.shape[1] - 3) % 3:
             raise ValueError("values must be [N][3*nD+3] float32 or float64")
         nD = (v.shape[1] - 3) // 3
         idx = C.c_int()
-        self._check(lib().parent_batch_add(self._h, nD, v.shape[0], int(v.dtype == np.float64), _ptr(v), 0,
-                                           C.byref(idx)))
+        if bins is None:
+            self._check(lib().parent_batch_add(self._h, nD, v.shape[0], int(v.dtype == np.float64), _ptr(v), 0,
+                                               C.byref(idx)))
+        else:
+            b = np.ascontiguousarray(bins, dtype=np.int32)
+            if b.size == 1:
+                b = np.full(6, int(b.reshape(-1)[0]), dtype=np.int32)
+            self._check(lib().parent_batch_add_binned(self._h, _ptr(b), nD, v.shape[0], int(v.dtype == np.float64),
+                                                      _ptr(v), 0, C.byref(idx)))
         self.n_dihedrals.append(nD)
         return idx.value
 

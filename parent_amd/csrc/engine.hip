@@ -738,6 +738,17 @@ int create_stream_and_events(parent_engine* e, int device) {
 
 int bfail(parent_batch* b, int code, const std::string& msg) { return fail(b->eng, code, msg); }
 
+// The element-wise largest bin counts of a batch's trajectories: what the
+// one kernel of a merged launch must hold (each tile reads its own
+// trajectory's counts from its descriptor).
+BinSet batch_envelope(const parent_batch* b) {
+    BinSet v;
+    for (const auto& it : b->items)
+        for (int k = 0; k < 6; k++) v.bins[k] = std::max(v.bins[k], it->set.bins[k]);
+    v.idx2 = bmax2d(v) > kMaxBins8 ? 2 : 1;
+    return v;
+}
+
 // Kernel choice for a whole batch.  One configuration runs every tile of the
 // merged queue, and the right one depends on the mix: per pair, a tile costs
 // a fixed part (clearing its LDS histograms, the epilogue) and a part per
@@ -753,7 +764,7 @@ int batch_autotune(parent_engine* e, parent_batch* b, int* method, int* variant)
         if (t.N > b->items[kl]->traj.N || (t.N == b->items[kl]->traj.N && t.D > b->items[kl]->traj.D)) kl = k;
         if (t.N < b->items[ks]->traj.N || (t.N == b->items[ks]->traj.N && t.D > b->items[ks]->traj.D)) ks = k;
     }
-    BinSet& s0 = b->items[kl]->set;
+    const BinSet s0 = batch_envelope(b);
     std::vector<std::pair<int, int>> cand;
     for (int id = 0; id < atomic_config_count(); id++)
         if (variant_fits(e, s0, id, b->items[kl]->traj.N)) cand.push_back({PARENT_METHOD_ATOMIC, id});
@@ -782,7 +793,7 @@ int batch_autotune(parent_engine* e, parent_batch* b, int* method, int* variant)
             HIPCHK(e, hipMemcpyAsync(dt.p, w.data(), w.size() * sizeof(int), hipMemcpyHostToDevice, e->stream));
             const PairInputs in = with_nlogn(e, pair_inputs(t, s));
             const TileQueue q{dt.as<int>(), (int)(w.size() / 2), 2, nullptr};
-            if (int rc = time_launch(e, [&] { return launch_tiled(e, in, bmax2d(s), m, id, q, nullptr, nullptr, nullptr); }, &ms[p]))
+            if (int rc = time_launch(e, [&] { return launch_tiled(e, in, bmax2d(s0), m, id, q, nullptr, nullptr, nullptr); }, &ms[p]))
                 return rc;
         }
         double alpha = 0, beta = 0;
@@ -1340,8 +1351,15 @@ const char* parent_batch_error(const parent_batch* b) { return b && b->eng ? b->
 
 int parent_batch_add(parent_batch* b, int n_dihedrals, int n_frames, int double_prec, const void* values,
                      int on_device, int* index) {
-    if (!b || !values) return PARENT_E_INVALID;
+    if (!b) return PARENT_E_INVALID;
+    return parent_batch_add_binned(b, b->bins, n_dihedrals, n_frames, double_prec, values, on_device, index);
+}
+
+int parent_batch_add_binned(parent_batch* b, const int bins[6], int n_dihedrals, int n_frames, int double_prec,
+                            const void* values, int on_device, int* index) {
+    if (!b || !values || !bins) return PARENT_E_INVALID;
     parent_engine* e = b->eng;
+    if (int rc = check_bins(e, bins)) return rc;
     if (n_dihedrals < 1 || n_frames < 1 || (double_prec != 0 && double_prec != 1))
         return fail(e, PARENT_E_INVALID, "need n_dihedrals >= 1, n_frames >= 1, double_prec in {0,1}");
     HIPCHK(e, hipSetDevice(e->device));
@@ -1360,7 +1378,7 @@ int parent_batch_add(parent_batch* b, int n_dihedrals, int n_frames, int double_
         HIPCHK(e, hipMemcpyAsync(t.values_own.p, values, bytes, hipMemcpyHostToDevice, e->stream));
         t.values = t.values_own.p;
     }
-    memcpy(it->set.bins, b->bins, sizeof(b->bins));
+    memcpy(it->set.bins, bins, sizeof(it->set.bins));
     // run_prepare may reallocate the shared n ln n table (a longer
     // trajectory), which the uploaded queue descriptors point to: drop the
     // queue before, so that a rejected trajectory leaves no stale pointer
@@ -1396,9 +1414,10 @@ int parent_batch_run(parent_batch* b, int method) {
     HIPCHK(e, launch_nlogn(nmax, e->scratch.nlogn.as<double>(), e->stream));
     BinSet& sb = b->items[big]->set;
     const Traj& tb = b->items[big]->traj;
+    const BinSet env = batch_envelope(b);  // the one kernel must hold every trajectory's bins
     int used = method, variant = 0;
     if (method < PARENT_METHOD_PLAIN || method > PARENT_METHOD_SLABS) return bfail(b, PARENT_E_INVALID, "unknown method");
-    if (method == PARENT_METHOD_SLABS || needs_slabs(e, sb)) {
+    if (method == PARENT_METHOD_SLABS || needs_slabs(e, env)) {
         // bins the LDS kernels cannot hold: the slab kernel, one launch per
         // trajectory (it has no tile queue)
         HIPCHK(e, hipEventRecord(e->ev[4], e->stream));
@@ -1430,7 +1449,7 @@ int parent_batch_run(parent_batch* b, int method) {
         }
         used = b->tuned_method;
         variant = b->tuned_variant;
-    } else if (method == PARENT_METHOD_MFMA && mfma_fits(e, sb)) {
+    } else if (method == PARENT_METHOD_MFMA && mfma_fits(e, env)) {
         used = PARENT_METHOD_MFMA;
         variant = 0;
     } else {
@@ -1438,11 +1457,11 @@ int parent_batch_run(parent_batch* b, int method) {
         // queue): the tiled atomic kernel, in a configuration whose counters
         // hold the longest trajectory
         used = PARENT_METHOD_ATOMIC;
-        variant = choose_atomic_config(e, sb, nmax);
+        variant = choose_atomic_config(e, env, nmax);
         if (variant < 0) return bfail(b, PARENT_E_UNSUPPORTED, "no tiled kernel fits these bins");
     }
     int tr, tc;
-    tile_shape(sb, used, variant, &tr, &tc);
+    tile_shape(env, used, variant, &tr, &tc);
     if (b->queue_items != n || b->queue_method != used || b->queue_variant != variant ||
         b->queue_nlogn != e->scratch.nlogn.p) {
         // Merged queue: items by decreasing frame count (cost per tile), each
@@ -1498,7 +1517,7 @@ int parent_batch_run(parent_batch* b, int method) {
     const PairInputs in0 = with_nlogn(e, pair_inputs(tb, sb));  // unused by the batch instantiations
     const TileQueue q{b->queue.as<int>(), b->ntiles, 3, b->descs.as<PairInputs>()};
     HIPCHK(e, hipEventRecord(e->ev[4], e->stream));
-    HIPCHK(e, launch_tiled(e, in0, bmax2d(sb), used, variant, q, nullptr, nullptr, nullptr));
+    HIPCHK(e, launch_tiled(e, in0, bmax2d(env), used, variant, q, nullptr, nullptr, nullptr));
     HIPCHK(e, hipEventRecord(e->ev[5], e->stream));
     HIPCHK(e, hipStreamSynchronize(e->stream));
     double upd = 0;

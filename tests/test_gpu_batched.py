@@ -186,3 +186,27 @@ def test_batch_rerun_after_rejected_longer_trajectory():
         for k in range(len(good)):
             s2 = b.results(k)[1]
             assert np.array_equal(s2.view(np.uint64), first[k].view(np.uint64)), f"trajectory {k}"
+
+
+@pytest.mark.parametrize("method", [0, 1, 2, 3])
+def test_batch_with_mixed_bins_in_one_launch(method):
+    """Trajectories with different bin counts in one merged-queue launch (the
+    kernel holds the batch's largest counts, each tile reads its own): every
+    trajectory against the oracle at its own bins and against the single
+    engine's S1D bit for bit."""
+    pa = _pa()
+    items = [((10, 1500, 1), [20] * 6), ((23, 4097, 2), [30] * 6), ((17, 800, 3), [7, 13, 29, 5, 40, 17]),
+             ((12, 3000, 4), [5] * 6), ((8, 2000, 5), [12, 9, 11, 10, 8, 12])]
+    trajs = [(pa.synth_values(a, n, s), bins) for (a, n, s), bins in items]
+    print(f"mixed-bins batch, method {method}", flush=True)
+    with pa.Batch([30] * 6) as b, pa.Engine(0) as single:
+        for v, bins in trajs:
+            b.add(v, bins)
+        b.run(method)
+        for k, (v, bins) in enumerate(trajs):
+            nD = v.shape[1] // 3 - 1
+            s1, s2 = b.results(k)
+            single.set_trajectory(v)
+            single.prepare(bins)
+            assert np.array_equal(s1.view(np.uint64), single.s1d().view(np.uint64))
+            _check_vs_oracle(nD, v, bins, s1, s2)
