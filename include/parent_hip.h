@@ -342,6 +342,11 @@ int parent_batch_add_binned(parent_batch* b, const int bins[6], int n_dihedrals,
  * PARENT_METHOD_PLAIN runs the tiled atomic kernel, as the plain kernel has
  * no tile queue).  stats.ms_pairs is that launch. */
 int parent_batch_run(parent_batch* b, int method);
+/* parent_engine_set_variant for the batch's kernels (method
+ * PARENT_METHOD_ATOMIC or PARENT_METHOD_SLABS): pins the configuration that
+ * parent_batch_run(b, PARENT_METHOD_ATOMIC) uses.  For testing; no
+ * counterpart in the reference. */
+int parent_batch_set_variant(parent_batch* b, int method, int variant);
 
 /* The batch autotuner (what PARENT_METHOD_AUTO runs on first use): every
  * configuration that fits is timed on the tuning rows of the longest and the

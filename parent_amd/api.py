@@ -63,6 +63,7 @@ def lib() -> C.CDLL:
         "parent_engine_mist_noalloc": (i32, [p, i32, p, p, p, p, p, p, p, p]),
         "parent_engine_sync": (i32, [p]),
         "parent_engine_autotune": (i32, [p, p, i32, C.POINTER(i32)]),
+        "parent_batch_set_variant": (i32, [p, i32, i32]),
         "parent_engine_set_variant": (i32, [p, i32, i32]),
         "parent_engine_variant_count": (i32, [p, i32, C.POINTER(i32)]),
         "parent_engine_stats": (i32, [p, p]),
@@ -463,6 +464,10 @@ class Batch:
                                                       _ptr(v), 0, C.byref(idx)))
         self.n_dihedrals.append(nD)
         return idx.value
+
+    def set_variant(self, method: int, variant: int) -> None:
+        """Pins the configuration `run(METHOD_ATOMIC)` uses (-1: default)."""
+        self._check(lib().parent_batch_set_variant(self._h, method, variant))
 
     def run(self, method: int = METHOD_AUTO) -> None:
         self._check(lib().parent_batch_run(self._h, method))

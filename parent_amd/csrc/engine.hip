@@ -1292,6 +1292,13 @@ int parent_engine_set_variant(parent_engine* e, int method, int variant) {
     return PARENT_OK;
 }
 
+int parent_batch_set_variant(parent_batch* b, int method, int variant) {
+    if (!b) return PARENT_E_INVALID;
+    const int rc = parent_engine_set_variant(b->eng, method, variant);
+    b->queue_items = -1;  // the merged queue is dealt for the kernel's launch size
+    return rc;
+}
+
 int parent_engine_variant_count(parent_engine* e, int method, int* count) {
     if (!e || !count) return PARENT_E_INVALID;
     *count = method == PARENT_METHOD_ATOMIC ? atomic_config_count()

@@ -71,7 +71,9 @@ __host__ __device__ __forceinline__ void pair_of_index(long long p, int D, int* 
 // of other workgroups on the CU.)
 template <int CTRL>
 __device__ __forceinline__ int dpp_mov(int v) {
-    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+    // every lane reads a lane of its own row, so no lane keeps its old value
+    // and none needs initialising
+    return __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, true);
 }
 template <int CTRL>
 __device__ __forceinline__ double dpp_mov(double v) {

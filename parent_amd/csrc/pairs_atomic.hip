@@ -80,6 +80,19 @@ __device__ __forceinline__ unsigned add_hi16(unsigned a, unsigned b) {
 
 // operand type of v_dot2_u32_u16 (__builtin_amdgcn_udot2)
 typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
+// halves added separately, modulo 2^16 each: one v_pk_add_u16.  Written
+// out: as 16-bit vector arithmetic the compiler splits the 16-byte LDS reads
+// that feed it into 12 + 4 bytes.
+__device__ __forceinline__ unsigned pk_add16(unsigned a, unsigned b) {
+    unsigned r;
+    asm("v_pk_add_u16 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+// Largest N at which a sum of 8 (4) 16-bit sub-counters of the sliced
+// kernels stays below 2^16: each holds at most N/16 + 16 counts.
+constexpr int kPk8Frames = (65535 / 8 - 16) * 16;
+constexpr int kPk4Frames = (65535 / 4 - 16) * 16;
 // an LDS word addressed by its 32-bit LDS byte address
 typedef __attribute__((address_space(3))) unsigned lds_u32;
 
@@ -836,7 +849,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(BLOCK / 1
         else return 1u << ((a & 1u) << 4);
     };
     auto first_words = [&](const PairInputs& in, const TileIdx& x, uint4& R, uint4& C) {
-        if (x.gi < x.gj && x.gj < in.D && in.N >= per_iter) {
+        if (x.gi < x.gj && x.gj < in.D && in.N > 0) {
             R = load16(index_row(in.bins + (size_t)x.gi * in.Np, in.Np), toff, 0);
             C = load16(index_row(in.bins + (size_t)x.gj * in.Np, in.Np), toff, 0);
         }
@@ -847,7 +860,15 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(BLOCK / 1
     PairInputs in = inputs(x);
     uint4 R0, C0;
     first_words(in, x, R0, C0);
-    for (int i = threadIdx.x; i < kSlotW / 4; i += BLOCK) h4[i] = make_uint4(0, 0, 0, 0);
+    {
+        // the clear unrolled: fixed LDS offsets instead of a counted loop
+        constexpr int kZ = kSlotW / 4;
+#pragma unroll
+        for (int r = 0; r < kZ / BLOCK; r++) h4[threadIdx.x + r * BLOCK] = make_uint4(0, 0, 0, 0);
+        if constexpr (kZ % BLOCK != 0) {
+            if ((int)threadIdx.x < kZ % BLOCK) h4[threadIdx.x + (kZ / BLOCK) * BLOCK] = make_uint4(0, 0, 0, 0);
+        }
+    }
     lds_barrier();
     int par = 0;
     // not unrolled: one tile's state at a time stays in registers
@@ -897,29 +918,43 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(BLOCK / 1
                                                __HIP_MEMORY_SCOPE_WORKGROUP);
                 }
             };
-            uint4 R1, C1;
+            // Iterations of per_iter frames; the last may be partial.  Its
+            // whole 16-frame chunks take the same path as the rest, and its
+            // one ragged chunk (N not a multiple of 16) goes frame by frame.
+            // Each iteration's words are requested one iteration ahead.
+            const int nit = (N + per_iter - 1) / per_iter;
+            uint4 R1 = make_uint4(0, 0, 0, 0), C1 = R1;
             int it = 0;
             for (; it + 2 <= nfull; it += 2) {
                 R1 = load16(rr, toff, (it + 1) * per_iter);
                 C1 = load16(cr, toff, (it + 1) * per_iter);
                 process(R0, C0);
-                if (it + 2 < nfull) {
-                    R0 = load16(rr, toff, (it + 2) * per_iter);
-                    C0 = load16(cr, toff, (it + 2) * per_iter);
-                }
+                // unconditional (past the row a buffer load reads zeros):
+                // without a branch the wait before process(R1) leaves these
+                // two loads in flight
+                R0 = load16(rr, toff, (it + 2) * per_iter);
+                C0 = load16(cr, toff, (it + 2) * per_iter);
                 process(R1, C1);
             }
-            if (it < nfull) process(R0, C0);
-            // the partial last iteration, one 4-frame index word per thread
-            const int nw = (N + 3) / 4;
-            for (int w = nfull * (per_iter / 4) + (int)threadIdx.x; w < nw; w += BLOCK) {
-                const unsigned rw = reinterpret_cast<const unsigned*>(rowp)[w];
-                const unsigned cw = reinterpret_cast<const unsigned*>(colp)[w];
-#pragma unroll
-                for (int e = 0; e < 4; e++) {
-                    if (4 * w + e >= N) break;
-                    const unsigned a = (rw >> (8 * e)) & 0xFFu, b = (cw >> (8 * e)) & 0xFFu;
-                    atomicAdd(&hist[cell_word(a, b, (unsigned)Bj)], cell_inc(a));
+            if (it < nfull) {
+                if (it + 1 < nit) {
+                    R1 = load16(rr, toff, (it + 1) * per_iter);
+                    C1 = load16(cr, toff, (it + 1) * per_iter);
+                }
+                process(R0, C0);
+                R0 = R1;
+                C0 = C1;
+                it++;
+            }
+            if (it < nit) {
+                const int left = N - it * per_iter - toff;
+                if (left >= 16) {
+                    process(R0, C0);
+                } else if (left > 0) {
+                    for (int e = 0; e < left; e++) {
+                        const unsigned a = byte_of(R0, e), b = byte_of(C0, e);
+                        atomicAdd(&hist[cell_word(a, b, (unsigned)Bj)], cell_inc(a));
+                    }
                 }
             }
         }
@@ -946,21 +981,52 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(BLOCK / 1
         }
         // ---- epilogue: read and zero this thread's groups
         const int ngroups = !valid ? 0 : PACK == 2 ? (Bi - 1) * (((Bj + 1) & ~1) / 2) + Bj : Bi * Bj;
+        // A 16-bit half holds at most N/16 + 16 counts (one sub-counter
+        // index), so halves of 8 (or 4) words add as packed 16-bit integers
+        // without carrying out when N is at most kPk8Frames (kPk4Frames).
+        const int pk = PACK == 2 && SL == 16 && valid ? (in.N <= kPk8Frames ? 8 : in.N <= kPk4Frames ? 4 : 1) : 1;
         unsigned cnt[2 * kMaxG];
 #pragma unroll
         for (int kk = 0; kk < kMaxG; kk++) {
             const int g = (int)threadIdx.x + kk * BLOCK;
             unsigned lo = 0, hi = 0;
             if (g < ngroups) {
+                uint4 v[NQ];
 #pragma unroll
                 for (int q4 = 0; q4 < NQ; q4++) {
-                    const uint4 v = h4[g * NQ + quad[q4]];
+                    v[q4] = h4[g * NQ + quad[q4]];
                     if (more) h4[g * NQ + quad[q4]] = make_uint4(0, 0, 0, 0);
-                    if constexpr (PACK == 1) {
-                        lo += (v.x + v.y) + (v.z + v.w);
+                }
+                if constexpr (PACK == 1) {
+#pragma unroll
+                    for (int q4 = 0; q4 < NQ; q4++) lo += (v[q4].x + v[q4].y) + (v[q4].z + v[q4].w);
+                } else if constexpr (SL == 16) {
+                    if (pk == 8) {
+                        const unsigned y0 = pk_add16(pk_add16(pk_add16(v[0].x, v[0].y), pk_add16(v[0].z, v[0].w)),
+                                                     pk_add16(pk_add16(v[1].x, v[1].y), pk_add16(v[1].z, v[1].w)));
+                        const unsigned y1 = pk_add16(pk_add16(pk_add16(v[2].x, v[2].y), pk_add16(v[2].z, v[2].w)),
+                                                     pk_add16(pk_add16(v[3].x, v[3].y), pk_add16(v[3].z, v[3].w)));
+                        lo = add_lo16(y0, y1);
+                        hi = add_hi16(y0, y1);
+                    } else if (pk == 4) {
+                        unsigned y[4];
+#pragma unroll
+                        for (int q4 = 0; q4 < 4; q4++)
+                            y[q4] = pk_add16(pk_add16(v[q4].x, v[q4].y), pk_add16(v[q4].z, v[q4].w));
+                        lo = add_lo16(y[0], y[1]) + add_lo16(y[2], y[3]);
+                        hi = add_hi16(y[0], y[1]) + add_hi16(y[2], y[3]);
                     } else {
-                        lo += add_lo16(v.x, v.y) + add_lo16(v.z, v.w);
-                        hi += add_hi16(v.x, v.y) + add_hi16(v.z, v.w);
+#pragma unroll
+                        for (int q4 = 0; q4 < NQ; q4++) {
+                            lo += add_lo16(v[q4].x, v[q4].y) + add_lo16(v[q4].z, v[q4].w);
+                            hi += add_hi16(v[q4].x, v[q4].y) + add_hi16(v[q4].z, v[q4].w);
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int q4 = 0; q4 < NQ; q4++) {
+                        lo += add_lo16(v[q4].x, v[q4].y) + add_lo16(v[q4].z, v[q4].w);
+                        hi += add_hi16(v[q4].x, v[q4].y) + add_hi16(v[q4].z, v[q4].w);
                     }
                 }
             }
@@ -1082,13 +1148,18 @@ struct AtomicVariant {
     int slot;  // words (kind 0) or cells (kind 1)
 };
 
+// tiles per workgroup of the k_pairs_multi kinds (1 for the others)
+constexpr int multi_tiles_per_workgroup(int kind) {
+    return kind == 5 ? 2 : 1;
+}
+
 // One launch of a configuration; batch queues (q.descs set) run the BATCH
 // instantiation, which reads each tile's inputs from q.descs.
 template <int KIND, int TR, int TC, int BLK, int SL>
 hipError_t launch_variant(dim3 grid, size_t lds, hipStream_t s, const PairInputs& in, const TileQueue& q, int t0,
                           int t1, const int* db, const int* de, int* dout) {
     if constexpr (KIND >= 5) {
-        constexpr int TPW = KIND == 5 ? 2 : 1;
+        constexpr int TPW = multi_tiles_per_workgroup(KIND);
         constexpr int NS = KIND == 7 ? 4 : 16;
         const void* mf = q.descs ? reinterpret_cast<const void*>(&k_pairs_multi<BLK, SL, 2, TPW, true, NS>)
                                  : reinterpret_cast<const void*>(&k_pairs_multi<BLK, SL, 2, TPW, false, NS>);
@@ -1154,9 +1225,10 @@ bool atomic_fits(int id, int bmax, size_t lds_max, int n_frames) {
     return cells <= v.slot && atomic_lds_bytes(id) <= lds_max;
 }
 
-// kind 5 covers 2 tiles per workgroup of a launch
+// the k_pairs_multi kinds with several tiles per workgroup cover as many
+// times the tiles of a launch
 int atomic_tiles_per_launch(int id) {
-    return tiles_per_launch(kAtomicVariants[id].cfg.block) * (kAtomicVariants[id].kind == 5 ? 2 : 1);
+    return tiles_per_launch(kAtomicVariants[id].cfg.block) * multi_tiles_per_workgroup(kAtomicVariants[id].kind);
 }
 
 hipError_t launch_pairs_atomic(const PairInputs& in, int id, const TileQueue& q,

@@ -131,6 +131,34 @@ def test_batch_of_uneven_trajectories(bins, method):
             assert np.max(np.abs(tot - tot1)) <= 1e-8
 
 
+@pytest.mark.parametrize("bins,variant", [([50] * 6, 53), ([30] * 6, 51), ([40] * 6, 54), ([75] * 6, 57)])
+def test_batch_on_pinned_variants(bins, variant):
+    """parent_batch_set_variant pins the merged launch's kernel: the multi-tile
+    kernels (one tile per 1,024-thread workgroup at 40-50 bins, two per
+    256-thread workgroup at 30, 4 sub-counters at 75) on trajectories of
+    different dof and frame counts: S2D within tolerance of the plain kernel
+    and of the oracle."""
+    pa = _pa()
+    shapes = [(10, 1500, 1), (23, 4097, 2), (17, 800, 3), (40, 3000, 4), (6, 5000, 5), (30, 20001, 6)]
+    trajs = [pa.synth_values(a, n, s) for a, n, s in shapes]
+    print(f"batch on pinned variant {variant}: bins={bins}", flush=True)
+    with pa.Batch(bins) as b, pa.Engine(0) as single:
+        for v in trajs:
+            b.add(v)
+        b.set_variant(pa.METHOD_ATOMIC, variant)
+        b.run(pa.METHOD_ATOMIC)
+        st = b.stats()
+        assert st["method"] == pa.METHOD_ATOMIC and st["kernel_variant"] == variant
+        for k, v in enumerate(trajs):
+            nD = v.shape[1] // 3 - 1
+            s1, s2 = b.results(k)
+            single.set_trajectory(v)
+            single.prepare(bins)
+            single.pairs(pa.METHOD_PLAIN)
+            assert np.max(np.abs(s2 - single.s2d())) <= S_ABS, f"trajectory {k}"
+            _check_vs_oracle(nD, v, bins, s1, s2)
+
+
 def test_batch_rejects_bad_data():
     pa = _pa()
     v = pa.synth_values(10, 500, 2)

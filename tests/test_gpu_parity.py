@@ -259,6 +259,40 @@ def test_mfma_out_of_range_falls_back(engine):
     assert engine.stats()["method"] == pa.METHOD_ATOMIC
 
 
+@pytest.mark.parametrize("variant,bins", [(53, 50), (51, 30)])
+@pytest.mark.parametrize("frames", [130800, 130801, 261872, 261873])
+def test_packed_sums_at_their_limits(engine, variant, bins, frames):
+    """The multi-tile kernel adds the 16-bit sub-counters of a cell as packed
+    halves, 8 at a time up to 130,800 frames and 4 at a time up to 261,872
+    (each sub-counter holds at most N/16 + 16 counts).  A constant trajectory
+    puts every frame in one cell: at each limit and one frame past it the
+    counts and entropies must match the plain kernel."""
+    pa = _pa()
+    nD = 3
+    D = 3 * nD + 3
+    row = np.array([0.12] * (nD + 2) + [1.9] * (nD + 1) + [3.0] * nD, dtype=np.float32)
+    vals = np.tile(row, (frames, 1))
+    vals[frames // 2, 0] = 0.5  # one frame elsewhere: two occupied cells in the bond pairs
+    engine.set_trajectory(vals)
+    engine.prepare([bins] * 6)
+    engine.pairs(pa.METHOD_PLAIN)
+    s2p = engine.s2d()
+    engine.set_variant(pa.METHOD_ATOMIC, variant)
+    engine.pairs(pa.METHOD_ATOMIC)
+    st = engine.stats()
+    print(f"packed sums: variant {variant} frames {frames} method {st['method']} variant {st['kernel_variant']}",
+          flush=True)
+    assert st["method"] == pa.METHOD_ATOMIC and st["kernel_variant"] == variant
+    assert np.max(np.abs(engine.s2d() - s2p)) <= S_ABS
+    pairs = [(0, 1), (0, D - 1), (D - 2, D - 1)]
+    h = engine.pair_counts(pairs, pa.METHOD_ATOMIC, [bins] * 3)
+    hp = engine.pair_counts(pairs, pa.METHOD_PLAIN, [bins] * 3)
+    engine.set_variant(pa.METHOD_ATOMIC, -1)
+    for x, xp in zip(h, hp):
+        assert np.array_equal(x, xp)
+        assert x.sum() == frames and x.max() >= frames - 1
+
+
 @pytest.mark.parametrize("variant,bins,slices", [(31, 50, 16), (45, 50, 16), (49, 30, 16), (53, 50, 16),
                                                  (51, 30, 16), (56, 100, 4), (57, 75, 4)])
 def test_packed_counters_at_the_frame_limit(engine, variant, bins, slices):
