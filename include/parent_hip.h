@@ -155,6 +155,7 @@ typedef struct {
     int kernel_variant;   /* tile configuration id chosen for that method */
     double ms_sweep;      /* last parent_engine_sweep: the graph launch (prepare + pairs of every set) */
     int sweep_sets;       /* number of bin sets in that sweep */
+    int mist_path;        /* last parent_engine_mist: 1 the spanning-tree path, 0 the O(V^2) Prim */
 } parent_stats;
 
 /* One row of the autotuner's table. */
@@ -297,6 +298,12 @@ int parent_engine_autotune(parent_engine* eng, parent_tune_entry* entries, int m
  * exercises several slabs at small bin counts.  For testing every
  * configuration the autotuner may pick; no counterpart in the reference. */
 int parent_engine_set_variant(parent_engine* eng, int method, int variant);
+/* How parent_engine_mist finds the tree: 0 (default) the spanning-tree path
+ * where it applies (dense MI matrix fits, V <= 16,384, no equal candidates
+ * in any Borůvka choice, every MI finite and > -1e200), else the O(V^2)
+ * Prim; 1 always the O(V^2) Prim.  Both give the same result.  For testing;
+ * no counterpart in the reference. */
+int parent_engine_set_mist_path(parent_engine* eng, int mode);
 int parent_engine_variant_count(parent_engine* eng, int method, int* count);
 
 /* Waits for the engine's stream. */

@@ -65,6 +65,7 @@ def lib() -> C.CDLL:
         "parent_engine_autotune": (i32, [p, p, i32, C.POINTER(i32)]),
         "parent_batch_set_variant": (i32, [p, i32, i32]),
         "parent_engine_set_variant": (i32, [p, i32, i32]),
+        "parent_engine_set_mist_path": (i32, [p, i32]),
         "parent_engine_variant_count": (i32, [p, i32, C.POINTER(i32)]),
         "parent_engine_stats": (i32, [p, p]),
         "parent_engine_sweep": (i32, [p, i32, p, i32]),
@@ -104,7 +105,8 @@ class _TuneEntry(C.Structure):
 class _Stats(C.Structure):
     _fields_ = [("ms_upload", C.c_double), ("ms_prepare", C.c_double), ("ms_pairs", C.c_double),
                 ("ms_mist", C.c_double), ("pair_updates", C.c_double), ("method", C.c_int),
-                ("kernel_variant", C.c_int), ("ms_sweep", C.c_double), ("sweep_sets", C.c_int)]
+                ("kernel_variant", C.c_int), ("ms_sweep", C.c_double), ("sweep_sets", C.c_int),
+                ("mist_path", C.c_int)]
 
 
 def _ptr(a: np.ndarray) -> C.c_void_p:
@@ -396,6 +398,10 @@ class Engine:
     def set_variant(self, method: int, variant: int) -> None:
         """Pins the configuration of `method` (-1: default)."""
         self._check(lib().parent_engine_set_variant(self._h, method, variant))
+
+    def set_mist_path(self, mode: int) -> None:
+        """0: the spanning-tree MIST path where it applies (default); 1: always the O(V^2) Prim."""
+        self._check(lib().parent_engine_set_mist_path(self._h, mode))
 
     def variant_count(self, method: int) -> int:
         n = C.c_int()

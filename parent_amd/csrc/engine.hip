@@ -107,7 +107,7 @@ struct BinSet {
 // (they run one after another on it).
 struct Scratch {
     DevBuf emin, emax, mfhist, nlogn;
-    DevBuf mdense, morder, mkey, mupos, mlive, medges, mmi, mleft, mflags;
+    DevBuf mdense, morder, mkey, mupos, mlive, medges, mmi, mleft, mflags, mtree;
 };
 
 }  // namespace
@@ -137,7 +137,8 @@ struct parent_engine {
     std::vector<Tuned> tuned;
     std::vector<parent_tune_entry> last_tune;
     int forced_atomic = -1;  // parent_engine_set_variant
-    int forced_slab_rows = 0;  // idem for PARENT_METHOD_SLABS: rows per slab (0: as many as the LDS holds)
+    int forced_slab_rows = 0;
+    int mist_path_mode = 0;    // parent_engine_set_mist_path  // idem for PARENT_METHOD_SLABS: rows per slab (0: as many as the LDS holds)
 
     // the captured sweep
     hipGraph_t graph = nullptr;
@@ -665,11 +666,23 @@ int run_mist(parent_engine* e, BinSet& s, int32_t* edges, double* mi, int32_t* l
         a.upos = x.mupos.as<unsigned>();
         a.live = x.mlive.as<int>();
     }
-    int flags = 0;
+    // the spanning-tree path first where it applies; the Prim kernel then
+    // returns at once if that path wrote the result
+    const bool tree = dense && e->mist_path_mode == 0 && V <= mist_reg_max_vertices() &&
+                      x.mtree.ensure(mist_tree_bytes(V)) == hipSuccess;
+    if (!tree) (void)hipGetLastError();
+    MistTreeArgs t{};
+    if (tree) {
+        t = mist_tree_layout(x.mtree.as<char>(), V, nullptr);
+        a.skip = t.state + 3;
+    }
+    int flags = 0, tree_done = 0;
     HIPCHK(e, hipEventRecord(e->ev[6], e->stream));
     if (dense) HIPCHK(e, launch_mi_dense(nD, a.s1d, a.s2d, x.mdense.as<double>(), e->stream));
+    if (tree) HIPCHK(e, launch_mist_tree(a, t, e->stream));
     HIPCHK(e, launch_mist(a, e->stream));
     HIPCHK(e, hipEventRecord(e->ev[7], e->stream));
+    if (tree) HIPCHK(e, hipMemcpyAsync(&tree_done, t.state + 3, sizeof(int), hipMemcpyDeviceToHost, e->stream));
     std::vector<double> s1(V);
     HIPCHK(e, hipMemcpyAsync(edges, a.edges, (size_t)(V - 1) * 16, hipMemcpyDeviceToHost, e->stream));
     HIPCHK(e, hipMemcpyAsync(mi, a.mi, (size_t)(V - 1) * 8, hipMemcpyDeviceToHost, e->stream));
@@ -678,6 +691,7 @@ int run_mist(parent_engine* e, BinSet& s, int32_t* edges, double* mi, int32_t* l
     HIPCHK(e, hipMemcpyAsync(&flags, x.mflags.p, sizeof(int), hipMemcpyDeviceToHost, e->stream));
     HIPCHK(e, hipStreamSynchronize(e->stream));
     e->stats.ms_mist = elapsed(e->ev[6], e->ev[7]);
+    e->stats.mist_path = tree_done ? 1 : 0;
     if (flags & kFlagMist) return fail(e, PARENT_E_DATA, "MIST: no finite mutual information left to pick");
     // totals in the reference's summation order (get_PAR_MIST.cpp:276-284, 663-741, 754-765)
     double t1[3] = {0, 0, 0};
@@ -1297,6 +1311,13 @@ int parent_batch_set_variant(parent_batch* b, int method, int variant) {
     const int rc = parent_engine_set_variant(b->eng, method, variant);
     b->queue_items = -1;  // the merged queue is dealt for the kernel's launch size
     return rc;
+}
+
+int parent_engine_set_mist_path(parent_engine* e, int mode) {
+    if (!e) return PARENT_E_INVALID;
+    if (mode != 0 && mode != 1) return fail(e, PARENT_E_INVALID, "MIST path mode must be 0 or 1");
+    e->mist_path_mode = mode;
+    return PARENT_OK;
 }
 
 int parent_engine_variant_count(parent_engine* e, int method, int* count) {

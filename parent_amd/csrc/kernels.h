@@ -131,7 +131,42 @@ struct MistArgs {
     double* key;             // [V] scratch, only when mist_keys_in_memory(V, mi_dense != nullptr)
     unsigned* upos;          // [V] idem
     int* live;               // [V] idem
+    const int* skip;         // or nullptr: nonzero when the tree path below has written the result
 };
+
+// Scratch of the tree path of MIST (launch_mist_tree), all device memory.
+struct MistTreeArgs {
+    int* comp;                  // [V] component of each vertex (a member's id)
+    int* hook;                  // [V] per component: the component its edge joins
+    int* hook2;                 // [V] idem after breaking mutual pairs, then the root
+    unsigned long long* vbest;  // [V] per vertex: best outgoing candidate (order image)
+    unsigned long long* vsec;   // [V] the second best
+    int* vnbr;                  // [V] the neighbour of the best
+    unsigned long long* cbest;  // [V] per component: the best of its vertices
+    int* ccount;                // [V] its vertices holding that best
+    int* csel;                  // [V] one such vertex
+    int* tree;                  // [2(V-1)] tree edges (x, y)
+    int* arc_off;               // [V+1] arcs by source vertex: offsets
+    int* arcs;                  // [2(V-1)] arc ids by source vertex
+    int* arc_pos;               // [2(V-1)] position of an arc in its source's list
+    int* succ;                  // [2][2(V-1)] Euler tour links, double-buffered
+    int* dist;                  // [2][2(V-1)] arcs to the end of the tour, double-buffered
+    int* parent;                // [V] the tree rooted at vertex 0
+    double* pmi;                // [V] MI(parent(v), v)
+    int* rank;                  // [V] rank of pmi[v] among the tree edges
+    int* state;                 // [8] 0 failed, 1 tree edges, 2 spanning, 3 result written, 4 rooted,
+                                //     5 two tree edges of equal MI
+};
+// Tree path of MIST (needs a.mi_dense, V <= mist_reg_max_vertices()): the
+// maximum spanning tree by Borůvka rounds, each component's edge required to
+// be its strictly largest outgoing candidate, then Prim over the tree's edges
+// only.  Writes a's outputs and sets t.state[3] when it succeeds; otherwise
+// leaves them to launch_mist (a.skip = t.state + 3).
+hipError_t launch_mist_tree(const MistArgs& a, const MistTreeArgs& t, hipStream_t s);
+// device bytes of MistTreeArgs for V vertices, and its pointers carved from
+// an allocation of that size at base
+size_t mist_tree_bytes(int V);
+MistTreeArgs mist_tree_layout(char* base, int V, size_t* bytes);
 // vertices whose keys one workgroup keeps in registers
 int mist_reg_max_vertices();
 // whether launch_mist needs MistArgs key/upos/live (V above the register form's limit)

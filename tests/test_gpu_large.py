@@ -4,7 +4,9 @@ resident in HBM.  Checked against the oracle on a seeded pair sample: every
 pair among a random selection of dofs (480 dofs by default, 114,960 pairs,
 0.1% of all pairs; PARENT_LARGE_SAMPLE_DOFS=1500 gives 1,124,250 pairs, 1%).
 Bars as for the small cases: |dS1D| <= 1e-12, |dS2D| <= 1e-10, MI within 1e-6
-relative with a 1e-11 floor; every one of the 112M S2D values finite.
+relative with a 1e-11 floor; every one of the 112M S2D values finite.  The
+MIST of the same entropies through the spanning-tree path and through the
+O(V^2) Prim: identical edges, MI bits, left counts and totals.
 """
 import os
 
@@ -34,6 +36,20 @@ def test_large_system_pair_sample():
         st = eng.stats()
         print(f"large: method {st['method']} variant {st['kernel_variant']} pairs {st['ms_pairs']:.1f} ms", flush=True)
         s1, s2 = eng.s1d(), eng.s2d()
+        # MIST of the 14,994 dofs: the spanning-tree path and the O(V^2) Prim agree bit for bit
+        runs = []
+        for mode in (0, 1):
+            eng.set_mist_path(mode)
+            r = eng.mist()
+            st = eng.stats()
+            runs.append((r, st["mist_path"]))
+            print(f"large: MIST mode {mode}: path {st['mist_path']}, {st['ms_mist']:.1f} ms", flush=True)
+        eng.set_mist_path(0)
+    (r0, p0), (r1, p1) = runs
+    assert p0 == 1 and p1 == 0
+    for x, y in zip(r0, r1):
+        assert np.array_equal(x.view(np.uint64) if x.dtype == np.float64 else x,
+                              y.view(np.uint64) if y.dtype == np.float64 else y)
     assert np.all(np.isfinite(s2))
     rng = np.random.default_rng(5)
     nsel = int(os.environ.get("PARENT_LARGE_SAMPLE_DOFS", "480"))

@@ -382,6 +382,87 @@ def test_mist_ties_between_signed_zeros(engine):
     assert np.array_equal(tot.view(np.uint64), to.view(np.uint64))
 
 
+def _mist_both_paths(engine, nD, s1, s2):
+    """parent_engine_mist through the spanning-tree path (default) and the
+    O(V^2) Prim: (results, path) of each."""
+    engine.set_entropies(nD, s1, s2)
+    engine.set_mist_path(0)
+    r0 = engine.mist()
+    p0 = engine.stats()["mist_path"]
+    engine.set_mist_path(1)
+    r1 = engine.mist()
+    p1 = engine.stats()["mist_path"]
+    engine.set_mist_path(0)
+    return r0, p0, r1, p1
+
+
+def _same_mist(a, b):
+    for x, y in zip(a, b):
+        assert np.array_equal(np.asarray(x).view(np.uint64) if x.dtype == np.float64 else x,
+                              np.asarray(y).view(np.uint64) if y.dtype == np.float64 else y)
+
+
+@pytest.mark.parametrize("nD,seed,scale", [(3, 1, 1.0), (17, 2, 1.0), (97, 3, 0.01), (330, 4, 1.0),
+                                           (1497, 5, 1.0)])
+def test_mist_tree_path_matches_prim(engine, nD, seed, scale):
+    """Continuous entropies (no equal MI): the spanning-tree path runs and gives
+    the O(V^2) Prim's edges, MI bits, left counts and totals; both match the
+    oracle's reference-order scan up to 300 dofs."""
+    D = 3 * nD + 3
+    rng = np.random.default_rng(seed)
+    s1 = rng.normal(size=D) * scale + 3.0
+    s2 = rng.normal(size=D * (D - 1) // 2) * scale + 6.0
+    print(f"mist tree path: {D} dofs", flush=True)
+    r0, p0, r1, p1 = _mist_both_paths(engine, nD, s1, s2)
+    assert p0 == 1 and p1 == 0
+    _same_mist(r0, r1)
+    if D <= 300:
+        _same_mist(r0, O.mist(nD, s1, s2))
+
+
+def test_mist_tree_path_on_trajectory_entropies(engine):
+    """Entropies of a synthetic trajectory (bins 50): tree path == Prim == oracle."""
+    vals, nD = _case(40, 3000, 9)
+    s1o, s2o, *_ = _oracle_full(vals, nD, [50] * 6)
+    r0, p0, r1, p1 = _mist_both_paths(engine, nD, s1o, s2o)
+    assert p0 == 1
+    _same_mist(r0, r1)
+    _same_mist(r0, O.mist(nD, s1o, s2o))
+
+
+@pytest.mark.parametrize("kind", ["ties", "signed_zeros", "nan", "no_candidate", "rounded"])
+def test_mist_tree_path_falls_back(engine, kind):
+    """Equal candidates in a Borůvka choice, NaN, MI <= -1e200: the tree path
+    gives up and the Prim's result (equal to the oracle's) stands.  Rounded
+    entropies (some equal MI values, not always in a choice) may take either
+    path; the result is the same."""
+    nD = 9
+    D = 3 * nD + 3
+    P = D * (D - 1) // 2
+    rng = np.random.default_rng(11)
+    s1 = rng.normal(size=D) + 3.0
+    s2 = rng.normal(size=P) + 6.0
+    if kind == "ties":
+        s1 = np.zeros(D)
+        s2 = -rng.integers(0, 3, P).astype(np.float64)
+    elif kind == "signed_zeros":
+        s1 = np.where(rng.integers(0, 2, D) == 1, -0.0, 0.0)
+        s2 = -rng.integers(0, 2, P).astype(np.float64)
+    elif kind == "nan":
+        s2[5] = np.nan
+    elif kind == "no_candidate":
+        s2[7] = 1e201  # MI = s1 + s1 - 1e201 < -1e200
+    else:
+        s1 = np.round(s1, 1)
+        s2 = np.round(s2, 1)
+    r0, p0, r1, p1 = _mist_both_paths(engine, nD, s1, s2)
+    print(f"mist fallback {kind}: path {p0}", flush=True)
+    if kind != "rounded":
+        assert p0 == 0
+    _same_mist(r0, r1)
+    _same_mist(r0, O.mist(nD, s1, s2))
+
+
 def test_drop_in_executables(tmp_path):
     """PARENT_hip.x / get_PAR_MIST_hip.x against the oracle's PARENT.x restatement."""
     pa = _pa()
