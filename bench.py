@@ -57,10 +57,14 @@ HBM_PEAK = 8.0e12
 LDS_PEAK_FALLBACK = {"conflict_free": 9.5e12, "random_2500": 5.2e12}  # lds_peak.x on MI355X
 # HBM bytes per pass of the pair kernel, from rocprofv3 FETCH_SIZE and WRITE_SIZE passes scaled
 # by the elementwise-kernel calibration (FETCH x 2.00, WRITE x 1.00):
-# profiles/pairs_multi_protein_1500x100k_b50.md (k_pairs_multi<1024, 1275, 2, 1>, the autotuner's
-# choice at 50 bins: 63.62 GB read + 0.166 GB written).
+# profiles/pairs_multi_protein_1500x100k_b50.md, tiles in bands of 32 row dofs
+# (k_pairs_multi<1024, 1275, 2, 1>, variant 53: 32.84 GB read + 0.80 GB written;
+# k_pairs_sliced<1, 1, 1024, 1275, 2, true>, variant 45: 32.09 GB + 0.80 GB; the autotuner picks
+# one of the two at 50 bins).
 # Keyed by (workload, method, variant): the figure is used only when the step ran that same kernel.
-PROFILED_TRAFFIC = {("protein_1500x100k_b50", 1, 53): (63.62e9 + 0.166e9,
+PROFILED_TRAFFIC = {("protein_1500x100k_b50", 1, 53): (32.84e9 + 0.80e9,
+                                                      "profiles/pairs_multi_protein_1500x100k_b50.md"),
+                    ("protein_1500x100k_b50", 1, 45): (32.09e9 + 0.80e9,
                                                       "profiles/pairs_multi_protein_1500x100k_b50.md")}
 METRIC = "pair-frame updates/s, all-pairs MI (4,491 dofs,100k fr,50 bins): tuned vs plain"
 

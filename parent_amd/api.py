@@ -390,10 +390,10 @@ class Engine:
 
     def autotune(self) -> list:
         """Times every pair-kernel configuration on a sample; returns the table."""
-        arr = (_TuneEntry * 64)()
+        arr = (_TuneEntry * 256)()
         n = C.c_int()
-        self._check(lib().parent_engine_autotune(self._h, C.cast(arr, C.c_void_p), 64, C.byref(n)))
-        return [{k: getattr(arr[i], k) for k, _ in _TuneEntry._fields_} for i in range(min(n.value, 64))]
+        self._check(lib().parent_engine_autotune(self._h, C.cast(arr, C.c_void_p), 256, C.byref(n)))
+        return [{k: getattr(arr[i], k) for k, _ in _TuneEntry._fields_} for i in range(min(n.value, 256))]
 
     def set_variant(self, method: int, variant: int) -> None:
         """Pins the configuration of `method` (-1: default)."""
@@ -480,10 +480,10 @@ class Batch:
 
     def autotune(self) -> list:
         """The batch autotuner's table (ms = predicted time of the whole queue)."""
-        arr = (_TuneEntry * 64)()
+        arr = (_TuneEntry * 256)()
         n = C.c_int()
-        self._check(lib().parent_batch_autotune(self._h, C.cast(arr, C.c_void_p), 64, C.byref(n)))
-        return [{k: getattr(arr[i], k) for k, _ in _TuneEntry._fields_} for i in range(min(n.value, 64))]
+        self._check(lib().parent_batch_autotune(self._h, C.cast(arr, C.c_void_p), 256, C.byref(n)))
+        return [{k: getattr(arr[i], k) for k, _ in _TuneEntry._fields_} for i in range(min(n.value, 256))]
 
     def results(self, k: int):
         """(s1d, s2d) of trajectory k."""
