@@ -13,7 +13,8 @@
 //     (one 16-byte store per dof and 16 frames) and each lane turns the 16
 //     index bytes of its operand row's dof into the row's one-hot bytes with
 //     one v_perm_b32 table lookup per 4 frames; frame f of the chunk is k
-//     slot f;
+//     slot f (from 9 bins, the rows of bins 8-11 read a second staged copy
+//     with the indices remapped, so that no lane masks its bytes);
 //   * 13 to 64 bins (PERM 0): every thread scatters the ones of its (dof,
 //     16-frame group) units into LDS operand tiles (one byte store per
 //     dof-frame) and clears its bytes again after the multiplies; frame
@@ -59,19 +60,25 @@ constexpr int kPermLds = kPermCountBytes > kPermStage ? kPermCountBytes : kPermS
 // The 64-bit v_perm_b32 table {hi, lo} whose byte lookup turns a word of 4
 // bin indices into the 4 one-hot bytes (index == k).  v_perm_b32 selector
 // bytes 0-7 pick bytes of {hi, lo} (0-3 from lo), 8-11 replicate the top bit
-// of byte 1, 3, 5 or 7, and 12 gives 0x00.  Bins 0-7 set byte k to 1; bins
-// 8-11 set the top bit of byte 1, 3, 5 or 7 and the caller keeps bit 0 of
-// every byte (& 0x01010101), which the byte-selected 0x80 values do not
-// have.  A row outside the tile (ok false) gets the zero table.
+// of byte 1, 3, 5 or 7, and 12 gives 0x00.  Bins 0-7 set byte k to 1; the
+// other staged values (bins 8-11, padding 12) select 0x00: the table has no
+// top bit set.  Rows of bins 8-11 read a second staged copy of the indices,
+// remapped so that bins 8-11 become selectors 0-3 and everything else 4-7
+// (remap_high_bins), with byte k - 8 of the table set.  A row outside the
+// tile (ok false) gets the zero table.
 __device__ __forceinline__ void onehot_table(bool ok, int k, unsigned* hi, unsigned* lo) {
     *hi = *lo = 0;
     if (!ok) return;
     if (k < 4) *lo = 1u << (8 * k);
     else if (k < 8) *hi = 1u << (8 * (k - 4));
-    else if (k == 8) *lo = 0x8000u;
-    else if (k == 9) *lo = 0x80000000u;
-    else if (k == 10) *hi = 0x8000u;
-    else *hi = 0x80000000u;
+    else *lo = 1u << (8 * (k - 8));
+}
+
+// Staged index bytes (bins 0-11, padding 12) -> selectors of the bin 8-11
+// rows: b - 8 for b in 8..11, (b & 7) | 4 (a zero byte of their tables) for
+// the rest.
+__device__ __forceinline__ unsigned remap_high_bins(unsigned x) {
+    return (x & 0x07070707u) | ((~x & 0x08080808u) >> 1);
 }
 
 // Workgroup barrier that waits for the wave's LDS operations only (global
@@ -84,7 +91,8 @@ __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(
 // PERM 1 (B <= 8) and 2 (B <= 12): each chunk's bin indices are staged in
 // LDS once, one 16-byte store per (dof, 16 frames), and every lane builds its
 // own MFMA operand row from them with one v_perm_b32 per 4 frames (PERM 2
-// adds a mask).  PERM 0: the one-hot bytes are scattered into LDS operand
+// stages a remapped second copy for the rows of bins 8-11).  PERM 0: the
+// one-hot bytes are scattered into LDS operand
 // tiles and cleared again (every B <= 64).  UPT (PERM only): staging units
 // per thread, ceil(2 * TD * 8 / 256); every thread issues all UPT loads, so
 // the waits for a staged chunk count the loads in flight exactly.  PERM
@@ -123,6 +131,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PERM ? (UPT
         // D-1, so the empty padding tiles of a batch queue (r0 = c0 = D)
         // read inside the index matrix too.
         const int side_bytes = TD * kIdxStride, buf_bytes = 2 * side_bytes;
+        // PERM 2: the remapped copy for the rows of bins 8-11, after both buffers
+        const int alt = PERM == 2 ? 2 * buf_bytes : 0;
         const int nunits = 2 * TD * 8;
         int ubase[UPT];
         const uint8_t* usrc[UPT];
@@ -155,6 +165,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PERM ? (UPT
                 uint4 x = src[k];
                 if (check) x = make_uint4(pad(x.x), pad(x.y), pad(x.z), pad(x.w));
                 *reinterpret_cast<uint4*>(smem + buf * buf_bytes + ubase[k]) = x;
+                if constexpr (PERM == 2)
+                    *reinterpret_cast<uint4*>(smem + alt + buf * buf_bytes + ubase[k]) =
+                        make_uint4(remap_high_bins(x.x), remap_high_bins(x.y), remap_high_bins(x.z),
+                                   remap_high_bins(x.w));
             }
         };
         // The lane's operand rows: A rows wr*64 + i*32 + lane%32 of the row
@@ -170,15 +184,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PERM ? (UPT
                 const int row = (s2 ? wc : wr) * 64 + i * 32 + (lane & 31);
                 const bool ok = row < TD * B;
                 onehot_table(ok, row % B, &thi[s2][i], &tlo[s2][i]);
-                toff[s2][i] = s2 * side_bytes + (ok ? row / B : 0) * kIdxStride + (lane >> 5) * 16;
+                toff[s2][i] = s2 * side_bytes + (ok ? row / B : 0) * kIdxStride + (lane >> 5) * 16 +
+                              (ok && row % B >= 8 ? alt : 0);
             }
         auto onehot = [&](int s2, int i, const uint4& x) {
             const unsigned words[4] = {x.x, x.y, x.z, x.w};
             v4i r;
 #pragma unroll
             for (int e = 0; e < 4; e++) {
-                const unsigned o = __builtin_amdgcn_perm(thi[s2][i], tlo[s2][i], words[e]);
-                r[e] = (int)(PERM == 2 ? o & 0x01010101u : o);
+                r[e] = (int)__builtin_amdgcn_perm(thi[s2][i], tlo[s2][i], words[e]);
             }
             return r;
         };
