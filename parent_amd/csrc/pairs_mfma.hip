@@ -66,12 +66,13 @@ constexpr int kPermLds = kPermCountBytes > kPermStage ? kPermCountBytes : kPermS
 // remapped so that bins 8-11 become selectors 0-3 and everything else 4-7
 // (remap_high_bins), with byte k - 8 of the table set.  A row outside the
 // tile (ok false) gets the zero table.
-__device__ __forceinline__ void onehot_table(bool ok, int k, unsigned* hi, unsigned* lo) {
-    *hi = *lo = 0;
-    if (!ok) return;
-    if (k < 4) *lo = 1u << (8 * k);
-    else if (k < 8) *hi = 1u << (8 * (k - 4));
-    else *lo = 1u << (8 * (k - 8));
+// Returned as {lo, hi} by value: through pointers the tables went to scratch
+// memory and every chunk then waited for all its global loads.
+__device__ __forceinline__ uint2 onehot_table(bool ok, int k) {
+    if (!ok) return make_uint2(0u, 0u);
+    if (k < 4) return make_uint2(1u << (8 * k), 0u);
+    if (k < 8) return make_uint2(0u, 1u << (8 * (k - 4)));
+    return make_uint2(1u << (8 * (k - 8)), 0u);
 }
 
 // Staged index bytes (bins 0-11, padding 12) -> selectors of the bin 8-11
@@ -183,7 +184,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PERM ? (UPT
             for (int i = 0; i < 2; i++) {
                 const int row = (s2 ? wc : wr) * 64 + i * 32 + (lane & 31);
                 const bool ok = row < TD * B;
-                onehot_table(ok, row % B, &thi[s2][i], &tlo[s2][i]);
+                const uint2 tab = onehot_table(ok, row % B);
+                tlo[s2][i] = tab.x;
+                thi[s2][i] = tab.y;
                 toff[s2][i] = s2 * side_bytes + (ok ? row / B : 0) * kIdxStride + (lane >> 5) * 16 +
                               (ok && row % B >= 8 ? alt : 0);
             }
