@@ -9,7 +9,7 @@ is uploaded to HBM before the timed region.
 Workloads (--workload):
   protein_1500x100k_b50   default: 1,500-atom chain, 4,494 dofs (10,095,771 pairs)
                           x 100,000 frames, 50 bins for every type
-  protein_1500x100k_bB    same at B = 5, 10, 20, 30, 75, 100 bins
+  protein_1500x100k_bB    same at B bins (listed: 5, 10, 20, 30, 75, 100; any B from 1 to 4,096)
   chain_100x10k_b50       100-atom chain, 294 dofs x 10,000 frames (the small case)
   large_5000x200k_b30     5,000-atom chain, 14,994 dofs (112,402,521 pairs) x 200,000 frames, 30 bins
   sweep_1500x100k         the default trajectory at B in {5,10,20,30,40,50,75,100}: one step = one
@@ -473,13 +473,26 @@ def run_readin(args, pa, nthreads):
          metric=".bat read-in (file -> frame-major values in HBM), bytes/s", unit="B/s")
 
 
+def workload_name(name):
+    """A listed workload, or the default trajectory at another bin count
+    (protein_1500x100k_bB, added to WORKLOADS)."""
+    if name in WORKLOADS or name in ("sweep_1500x100k", "batch64_b30", "readin_1500x100k"):
+        return name
+    if name.startswith("protein_1500x100k_b") and name[19:].isdigit() and 1 <= int(name[19:]) <= 4096:
+        WORKLOADS[name] = (1500, 100_000, int(name[19:]))
+        return name
+    raise argparse.ArgumentTypeError(f"unknown workload {name}")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="protein_1500x100k_b50",
-                    choices=sorted(WORKLOADS) + ["sweep_1500x100k", "batch64_b30", "readin_1500x100k"])
+    ap.add_argument("--workload", default="protein_1500x100k_b50", type=workload_name,
+                    help="one of " + ", ".join(sorted(WORKLOADS) + ["sweep_1500x100k", "batch64_b30",
+                                                                   "readin_1500x100k"])
+                    + "; protein_1500x100k_bB also takes any other bin count B from 1 to 4,096")
     ap.add_argument("--method", default="auto", choices=["auto", "plain", "atomic", "mfma"])
     ap.add_argument("--plain-steps", type=int, default=1, help="timed steps of the plain kernel (0: skip)")
     ap.add_argument("--cpu-sample-atoms", type=int, default=500)

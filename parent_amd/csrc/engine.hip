@@ -73,7 +73,7 @@ struct DevBuf {
 };
 
 struct TileList {
-    int tr = -1, tc = -1, D = -1;
+    int tr = -1, tc = -1, D = -1, Np = -1;  // the list's tile shape, dofs and band (by Np)
     int ntiles = 0;
     DevBuf dev;
 };
@@ -319,8 +319,10 @@ int run_prepare(parent_engine* e, const Traj& t, BinSet& s) {
 // within a band column by column, so the workgroups running at the same time
 // (one contiguous run of the list per XCD) read the same column-dof index rows
 // from L2 while their band's row-dof rows stay resident there.  A taller band
-// re-reads each column row from HBM fewer times: at 100k frames 32 rows hold
-// 3.2 MB of the XCD's 4 MB L2 (16 and 48 rows ran within 1% of the time).
+// re-reads each column row from HBM fewer times, as long as the band's rows fit
+// the XCD's 4 MB L2 beside the columns in flight: bands of about 3.2 MB of
+// index rows, 8 to 32 rows (32 at 100k frames: half the HBM reads of 16 rows;
+// 16 at 200k frames, where 32 rows of 6.4 MB ran 1.7% slower).
 void push_band(std::vector<int>& t, int D, int tr, int tc, int b0, int b1) {
     const int cfirst = ((b0 + 1) / tc) * tc;
     for (int c0 = cfirst; c0 < D; c0 += tc)
@@ -331,18 +333,21 @@ void push_band(std::vector<int>& t, int D, int tr, int tc, int b0, int b1) {
             }
 }
 
-constexpr int kTileBand = 32;
-std::vector<int> make_tiles(int D, int tr, int tc) {
+int tile_band(int tr, int Np) {
+    const int rows = std::min(32, std::max(8, (int)(3276800LL / std::max(Np, 1))));
+    return std::max(tr, rows / tr * tr);
+}
+std::vector<int> make_tiles(int D, int Np, int tr, int tc) {
     std::vector<int> t;
-    const int band = std::max(tr, kTileBand / tr * tr);
+    const int band = tile_band(tr, Np);
     for (int b0 = 0; b0 < D - 1; b0 += band) push_band(t, D, tr, tc, b0, std::min(b0 + band, D - 1));
     return t;
 }
 
-// Number of tiles make_tiles(D, tr, tc) lists, without building the list.
-long long count_tiles(int D, int tr, int tc) {
+// Number of tiles make_tiles(D, Np, tr, tc) lists, without building the list.
+long long count_tiles(int D, int Np, int tr, int tc) {
     long long n = 0;
-    const int band = std::max(tr, kTileBand / tr * tr);
+    const int band = tile_band(tr, Np);
     for (int b0 = 0; b0 < D - 1; b0 += band) {
         const int b1 = std::min(b0 + band, D - 1);
         const int cfirst = ((b0 + 1) / tc) * tc;
@@ -370,8 +375,8 @@ void tile_shape(const BinSet& s, int method, int variant, int* tr, int* tc) {
 int ensure_tiles(parent_engine* e, const Traj& t, BinSet& s, int method, int variant) {
     int tr, tc;
     tile_shape(s, method, variant, &tr, &tc);
-    if (s.tiles.tr == tr && s.tiles.tc == tc && s.tiles.D == t.D) return PARENT_OK;
-    std::vector<int> tl = make_tiles(t.D, tr, tc);
+    if (s.tiles.tr == tr && s.tiles.tc == tc && s.tiles.D == t.D && s.tiles.Np == t.Np) return PARENT_OK;
+    std::vector<int> tl = make_tiles(t.D, t.Np, tr, tc);
     HIPCHK(e, s.tiles.dev.ensure(std::max<size_t>(tl.size(), 3) * sizeof(int)));
     if (!tl.empty())
         HIPCHK(e, hipMemcpyAsync(s.tiles.dev.p, tl.data(), tl.size() * sizeof(int), hipMemcpyHostToDevice, e->stream));
@@ -379,6 +384,7 @@ int ensure_tiles(parent_engine* e, const Traj& t, BinSet& s, int method, int var
     s.tiles.tr = tr;
     s.tiles.tc = tc;
     s.tiles.D = t.D;
+    s.tiles.Np = t.Np;
     s.tiles.ntiles = (int)(tl.size() / 2);
     return PARENT_OK;
 }
@@ -831,7 +837,7 @@ int batch_autotune(parent_engine* e, parent_batch* b, int* method, int* variant)
             }
         }
         double total_tiles = 0;
-        for (auto& it : b->items) total_tiles += (double)count_tiles(it->traj.D, tr, tc);
+        for (auto& it : b->items) total_tiles += (double)count_tiles(it->traj.D, it->traj.Np, tr, tc);
         const double pred = alpha * total_tiles + beta * total_upd;
         const int block = m == PARENT_METHOD_MFMA ? 256 : atomic_config(id).block;
         e->last_tune.push_back(parent_tune_entry{m, id, tr, tc, block, pred, total_upd, total_upd / (pred * 1e-3)});
@@ -1506,7 +1512,7 @@ int parent_batch_run(parent_batch* b, int method) {
         std::vector<int> lst;
         for (int k : order) {
             const Traj& t = b->items[k]->traj;
-            std::vector<int> tl = make_tiles(t.D, tr, tc);
+            std::vector<int> tl = make_tiles(t.D, t.Np, tr, tc);
             for (size_t i = 0; i + 1 < tl.size(); i += 2) {
                 lst.push_back(tl[i]);
                 lst.push_back(tl[i + 1]);
