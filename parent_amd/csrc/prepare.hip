@@ -384,6 +384,28 @@ __device__ __forceinline__ void row_histogram(const I* __restrict__ row, int Np,
     }
 }
 
+// The same into 16 lane-sliced copies, h[b * 16 + lane % 16], for bin counts
+// up to 16: the 16 lanes the LDS serves together never share a word, where a
+// single histogram of a few bins serialises most of a wave's atomics (5 bins:
+// 0.98 ms for the 1D pass of 4,494 dofs x 100k frames).
+template <class I>
+__device__ __forceinline__ void row_histogram_sliced(const I* __restrict__ row, int Np, unsigned* h) {
+    constexpr int per = 16 / (int)sizeof(I);
+    constexpr unsigned pad = (unsigned)(I)~(I)0;
+    const uint4* r = reinterpret_cast<const uint4*>(row);
+    unsigned* hl = h + (threadIdx.x & 15);
+    for (int i = threadIdx.x; i < Np / per; i += 256) {
+        const uint4 w = r[i];
+        const unsigned ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+        for (int q = 0; q < per; q++) {
+            const unsigned b = sizeof(I) == 1 ? (ws[q >> 2] >> (8 * (q & 3))) & 0xFFu
+                                              : (ws[q >> 1] >> (16 * (q & 1))) & 0xFFFFu;
+            if (b != pad) atomicAdd(&hl[b * 16], 1u);
+        }
+    }
+}
+
 // Pass 6: per-dof histograms, the 1D entropy with the reference's formula
 // (PARENT.cpp:650-671 bonds, 694-715 angles, 738-757 dihedrals), and the
 // per-dof constants of the 2D epilogue:
@@ -415,10 +437,28 @@ __global__ __launch_bounds__(256) void k_dof_stats(const void* __restrict__ idx2
     const int g = blockIdx.x;
     const int t = dof_type(nD, g);
     const bool sep1 = idx1 != nullptr && B1.of(t) != B2.of(t);
+    // counts of up to 16 bins go through lane-sliced copies, then add up in h2 / h1
+    __shared__ unsigned hs[2][16 * 16];
+    const bool sl2 = nb2 <= 16, sl1 = sep1 && nh1 <= 16;
     for (int i = threadIdx.x; i < nb2 + nh1; i += 256) h2[i] = 0;
+    for (int i = threadIdx.x; i < 2 * 16 * 16; i += 256) hs[i >> 8][i & 255] = 0;
     __syncthreads();
-    row_histogram(static_cast<const I2*>(idx2) + (size_t)g * Np, Np, h2);
-    if (sep1) row_histogram(static_cast<const I1*>(idx1) + (size_t)g * Np, Np, h1);
+    if (sl2) row_histogram_sliced(static_cast<const I2*>(idx2) + (size_t)g * Np, Np, hs[0]);
+    else row_histogram(static_cast<const I2*>(idx2) + (size_t)g * Np, Np, h2);
+    if (sep1) {
+        if (sl1) row_histogram_sliced(static_cast<const I1*>(idx1) + (size_t)g * Np, Np, hs[1]);
+        else row_histogram(static_cast<const I1*>(idx1) + (size_t)g * Np, Np, h1);
+    }
+    if (sl2 || sl1) {
+        __syncthreads();
+        const int k = threadIdx.x & 15, side = threadIdx.x >> 4;  // threads 0-31: (bin, histogram)
+        if (side < 2 && (side ? sl1 && k < nh1 : sl2 && k < nb2)) {
+            unsigned n = 0;
+#pragma unroll
+            for (int l = 0; l < 16; l++) n += hs[side][k * 16 + l];
+            (side ? h1 : h2)[k] = n;
+        }
+    }
     const unsigned* histo = sep1 ? h1 : h2;
     const int numFrames = N;
     const int bins = B1.of(t), bins2 = B2.of(t);
