@@ -496,7 +496,7 @@ int run_autotune(parent_engine* e, const Traj& tj, BinSet& s) {
     // candidates: every fitting atomic configuration, then the MFMA kernel
     std::vector<std::pair<int, int>> cand;
     for (int id = 0; id < atomic_config_count(); id++)
-        if (variant_fits(e, s, id, tj.N)) cand.push_back({PARENT_METHOD_ATOMIC, id});
+        if (atomic_tuned(id) && variant_fits(e, s, id, tj.N)) cand.push_back({PARENT_METHOD_ATOMIC, id});
     if (mfma_fits(e, s)) cand.push_back({PARENT_METHOD_MFMA, 0});
     auto measure = [&](int m, int id, int runs, int tiles_per_cu, int reps, parent_tune_entry* te) -> int {
         int tr, tc;
@@ -790,7 +790,7 @@ int batch_autotune(parent_engine* e, parent_batch* b, int* method, int* variant)
     const BinSet s0 = batch_envelope(b);
     std::vector<std::pair<int, int>> cand;
     for (int id = 0; id < atomic_config_count(); id++)
-        if (variant_fits(e, s0, id, b->items[kl]->traj.N)) cand.push_back({PARENT_METHOD_ATOMIC, id});
+        if (atomic_tuned(id) && variant_fits(e, s0, id, b->items[kl]->traj.N)) cand.push_back({PARENT_METHOD_ATOMIC, id});
     if (mfma_fits(e, s0)) cand.push_back({PARENT_METHOD_MFMA, 0});
     if (cand.empty()) return bfail(b, PARENT_E_UNSUPPORTED, "no tiled kernel fits these bins");
     double total_upd = 0;

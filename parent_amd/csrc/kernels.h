@@ -84,6 +84,8 @@ struct AtomicConfig {
     int block;     // threads per workgroup
 };
 int atomic_config_count();
+// whether the autotuners time configuration id
+bool atomic_tuned(int id);
 AtomicConfig atomic_config(int id);
 // 0 plain LDS histograms, 1 sliced (bank-conflict-free), 2 packed sliced,
 // 3 sliced with the perm + dot2 fill, 4 interleaved packed with that fill,

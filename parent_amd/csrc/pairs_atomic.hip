@@ -1202,6 +1202,13 @@ const AtomicVariant kAtomicVariants[] = {PARENT_ATOMIC_VARIANTS(PARENT_ATOMIC_EN
 }  // namespace
 
 int atomic_config_count() { return (int)(sizeof(kAtomicVariants) / sizeof(kAtomicVariants[0])); }
+// Variant 45 (k_pairs_sliced, 1,024 threads, interleaved halves up to 50
+// bins) stays out of the autotuners' candidates: on their samples it times
+// up to 1% ahead of variant 53 (k_pairs_multi, the same layout and fill),
+// and over whole passes it runs 1.3% behind it (163.3 against 161.0 ms at
+// 4,494 dofs x 100k frames x 50 bins, on samples up to 3.4M tiles as well).
+// It remains selectable with parent_engine_set_variant.
+bool atomic_tuned(int id) { return id != 45; }
 AtomicConfig atomic_config(int id) { return kAtomicVariants[id].cfg; }
 int atomic_kind(int id) { return kAtomicVariants[id].kind; }
 size_t atomic_lds_bytes(int id) {
