@@ -210,9 +210,10 @@ MFMA_CASES = [
     ([3, 9, 12, 5, 7, 11], 24, 2500),   # mixed types: 12-row dof slots, per-type cells
     ([20] * 6, 16, 17000),              # frames not a multiple of the 128-frame chunk
     ([40] * 6, 14, 2000),
-    ([8] * 6, 28, 2049),                # register-built operands: 8 bins without a mask,
-    ([9] * 6, 22, 3000),                # 9 and 11 bins with the mask, partial last chunks
-    ([11] * 6, 20, 1100),
+    ([8] * 6, 28, 2049),                # register-built operands: 8 bins from one staged copy,
+    ([9] * 6, 22, 3000),                # 9, 11 and 12 bins with the bins 8-11 rows on the
+    ([11] * 6, 20, 1100),               # remapped copy, partial last chunks
+    ([12] * 6, 18, 1537),
 ]
 
 
