@@ -202,6 +202,39 @@ def test_every_atomic_variant_bit_exact(engine):
         assert tested >= 1
 
 
+@pytest.mark.parametrize("atoms,frames", [(6, 1), (6, 2), (8, 5), (10, 16), (10, 17), (12, 63), (14, 200),
+                                           (8, 1025)])
+def test_short_trajectories_every_kernel(engine, atoms, frames):
+    """Trajectories shorter than one 16-frame chunk, one wave of chunks or one
+    iteration of the 1,024-thread fills: every atomic variant that fits, the
+    MFMA, plain and slab kernels, against the oracle's S2D and a pair's counts."""
+    pa = _pa()
+    vals, nD = _case(atoms, frames, 29)
+    D = 3 * nD + 3
+    for bins in ([50] * 6, [5] * 6, [7, 13, 29, 5, 40, 17]):
+        print(f"short: atoms={atoms} frames={frames} bins={bins}", flush=True)
+        engine.set_trajectory(vals)
+        engine.prepare(bins)
+        s1o, s2o, mno, mxo, _, shifted = _oracle_full(vals, nD, bins)
+        assert np.max(np.abs(engine.s1d() - s1o)) <= 1e-12
+        t0, t1 = O.type_of(nD, 0), O.type_of(nD, D - 1)
+        ho = O.entropy2d(t0, shifted[0], mno[0], mxo[0], bins[3 + t0],
+                         t1, shifted[D - 1], mno[D - 1], mxo[D - 1], bins[3 + t1], counts=True)[1]
+        runs = [(pa.METHOD_PLAIN, -1), (pa.METHOD_MFMA, -1), (pa.METHOD_SLABS, -1)]
+        runs += [(pa.METHOD_ATOMIC, v) for v in range(engine.variant_count(pa.METHOD_ATOMIC))]
+        for method, v in runs:
+            if v >= 0:
+                engine.set_variant(pa.METHOD_ATOMIC, v)
+            engine.pairs(method)
+            st = engine.stats()
+            if v >= 0 and st["method"] != pa.METHOD_ATOMIC:
+                continue  # histograms do not fit this variant's slots
+            assert np.max(np.abs(engine.s2d() - s2o)) <= S_ABS, f"method {method} variant {v}"
+            h = engine.pair_counts([(0, D - 1)], st["method"], bins[3:])[0]
+            assert np.array_equal(h.astype(np.int64), ho), f"method {method} variant {v}"
+        engine.set_variant(pa.METHOD_ATOMIC, -1)
+
+
 MFMA_CASES = [
     ([5] * 6, 30, 3001),
     ([10] * 6, 26, 4097),
