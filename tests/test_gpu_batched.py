@@ -131,6 +131,25 @@ def test_batch_of_uneven_trajectories(bins, method):
             assert np.max(np.abs(tot - tot1)) <= 1e-8
 
 
+@pytest.mark.parametrize("method", [1, 2, 3])  # atomic, MFMA, auto
+def test_batch_with_short_trajectories(method):
+    """Trajectories of 1 to 17 frames beside longer ones in one launch: the
+    short ones' tiles hold fewer frames than one 16-frame chunk of a thread."""
+    pa = _pa()
+    bins = [10] * 6 if method == 2 else [30] * 6
+    shapes = [(8, 1, 7), (12, 2, 8), (9, 5, 9), (14, 17, 10), (20, 3000, 11), (6, 16, 12)]
+    trajs = [pa.synth_values(a, n, s) for a, n, s in shapes]
+    print(f"short batch bins={bins} method={method}", flush=True)
+    with pa.Batch(bins) as b:
+        for v in trajs:
+            b.add(v)
+        b.run(method)
+        for k, v in enumerate(trajs):
+            nD = v.shape[1] // 3 - 1
+            s1, s2 = b.results(k)
+            _check_vs_oracle(nD, v, bins, s1, s2)
+
+
 @pytest.mark.parametrize("bins,variant", [([50] * 6, 53), ([30] * 6, 51), ([40] * 6, 54), ([75] * 6, 57)])
 def test_batch_on_pinned_variants(bins, variant):
     """parent_batch_set_variant pins the merged launch's kernel: the multi-tile
