@@ -136,65 +136,114 @@ __global__ __launch_bounds__(256) void k_scan_raw(const T* __restrict__ v, int N
     }
 }
 
+// The empty-bin scans of one dihedral's 100-bin histogram (PARENT.cpp:573-607),
+// done by one wave over ballot masks instead of two serial passes per thread.
+// Returns the modfit bin: with an empty bin, the start of the longest run of
+// empty bins; otherwise the first of the least populated bins.
+//
+// The reference walks k = 0 ... 199 over bin k % 100 and keeps a run only
+// when it is strictly longer than the longest so far, so among the longest
+// runs the one that ends first wins.  Every run but one ends in the order of
+// its start bin; the run that holds bins 99 and 0 ends last, in the second
+// pass, and the walk's first run, which begins at bin 0, is then a strict
+// part of it and never wins.  So: the first start bin (a run start is an
+// empty bin after an occupied one, circularly) of the longest runs, lengths
+// counted circularly.  No occupied bin at all leaves the reference's -1.
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) {
+        const unsigned lo = __shfl_xor((unsigned)v, m), hi = __shfl_xor((unsigned)(v >> 32), m);
+        const unsigned long long o = ((unsigned long long)hi << 32) | lo;
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
+__device__ __forceinline__ int trailing_ones(unsigned long long x) {
+    return x == ~0ull ? 64 : __builtin_ctzll(~x);
+}
+
+__device__ int modfit_bin(const unsigned* __restrict__ histo, int lane) {
+    static_assert(kModfitBins > 64 && kModfitBins <= 128, "two ballot words per histogram");
+    constexpr int kHigh = kModfitBins - 64;
+    const unsigned c0 = histo[lane], c1 = lane < kHigh ? histo[64 + lane] : 1u;
+    // bit k of z0 (bins 0-63) and of z1 (bins 64-99): bin empty
+    const unsigned long long z0 = __ballot(c0 == 0u), z1 = __ballot(lane < kHigh && c1 == 0u);
+    if (z0 == 0 && z1 == 0) {
+        // key = ~count : ~bin, so the largest key is the lowest count's first bin
+        unsigned long long key = ((unsigned long long)~c0 << 32) | (unsigned)~lane;
+        if (lane < kHigh) {
+            const unsigned long long k1 = ((unsigned long long)~c1 << 32) | (unsigned)~(64 + lane);
+            key = k1 > key ? k1 : key;
+        }
+        return (int)~(unsigned)(wave_max_u64(key) & 0xFFFFFFFFull);
+    }
+    if (z0 == ~0ull && z1 == (1ull << kHigh) - 1) return -1;
+    auto empty = [&](int k) { return (int)((k < 64 ? z0 >> k : z1 >> (k - 64)) & 1u); };
+    // empty bins from bin k on, up to bin 99 (z1 holds no bits past it)
+    auto empties_from = [&](int k) {
+        if (k >= 64) return trailing_ones(z1 >> (k - 64));
+        const int t = trailing_ones(z0 >> k);
+        return t == 64 - k ? t + trailing_ones(z1) : t;
+    };
+    // key = run length : ~start bin, so the largest key is the longest run's first start
+    unsigned long long key = 0;
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const int k = lane + 64 * h;
+        if (k < kModfitBins && empty(k) && !empty(k == 0 ? kModfitBins - 1 : k - 1)) {
+            int len = empties_from(k);
+            if (k + len == kModfitBins) len += empties_from(0);
+            const unsigned long long kk = ((unsigned long long)len << 32) | (unsigned)~k;
+            key = kk > key ? kk : key;
+        }
+    }
+    return (int)~(unsigned)(wave_max_u64(key) & 0xFFFFFFFFull);
+}
+
 // Pass 2: range rules of bonds and angles, and the modfit shift of each
 // dihedral from its 100-bin histogram (PARENT.cpp:522-533, 546-562, 573-609).
-// One thread per dof, kFinalizeBlock dofs a block.
-constexpr int kFinalizeBlock = 64;
-__global__ __launch_bounds__(kFinalizeBlock) void k_finalize_ranges(int D, int nD, double pi, double mf_bin,
+// kFinalizeBlock dofs a block: all threads stage the block's histograms in
+// LDS, each wave then takes every fourth dihedral of the block, and one
+// thread per bond or angle applies its rules.
+constexpr int kFinalizeBlock = 16, kFinalizeThreads = 256;
+__global__ __launch_bounds__(kFinalizeThreads) void k_finalize_ranges(int D, int nD, double pi, double mf_bin,
                                   const unsigned long long* __restrict__ emin,
                                   const unsigned long long* __restrict__ emax,
                                   const unsigned* __restrict__ mfh, double* vmin, double* vmax,
                                   double* modfit, int* flags) {
-    // The block's dihedral histograms are staged in LDS with coalesced loads
-    // first: the serial scans below then read LDS, not global memory.
-    __shared__ unsigned sh[kFinalizeBlock * kModfitBins];
+    // coalesced staging, all of a thread's loads issued before the first store
+    constexpr int kWords = kFinalizeBlock * kModfitBins, kPerThread = (kWords + kFinalizeThreads - 1) / kFinalizeThreads;
+    __shared__ unsigned sh[kWords];
     const int g0 = blockIdx.x * kFinalizeBlock, dih0 = 2 * nD + 3;
-    for (int i = threadIdx.x; i < kFinalizeBlock * kModfitBins; i += kFinalizeBlock) {
-        const int gg = g0 + i / kModfitBins;
-        if (gg >= dih0 && gg < D) sh[i] = mfh[(size_t)(gg - dih0) * kModfitBins + i % kModfitBins];
+    const bool blk_dih = g0 + kFinalizeBlock > dih0;
+    if (blk_dih) {
+        unsigned w[kPerThread];
+#pragma unroll
+        for (int u = 0; u < kPerThread; u++) {
+            const int i = threadIdx.x + u * kFinalizeThreads, gg = g0 + i / kModfitBins;
+            w[u] = i < kWords && gg >= dih0 && gg < D ? mfh[(size_t)(gg - dih0) * kModfitBins + i % kModfitBins] : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < kPerThread; u++) {
+            const int i = threadIdx.x + u * kFinalizeThreads;
+            if (i < kWords) sh[i] = w[u];
+        }
     }
     __syncthreads();
-    const int g = g0 + threadIdx.x;
-    if (g >= D) return;
-    const int t = dof_type(nD, g);
-    if (t == 2) {
-        const unsigned* histo = sh + threadIdx.x * kModfitBins;
-        bool zeroExists = false;
-        for (int k = 0; k < kModfitBins; k++) zeroExists = zeroExists || (histo[k] == 0);
-        long long longest;
-        int longestPos, current, currentPos = 0;
-        if (zeroExists) {
-            longest = 0;
-            current = 0;
-            longestPos = -1;
-            for (int k = 0; k < 2 * kModfitBins; k++) {
-                const int l = k % kModfitBins;
-                if ((current == 0) && (histo[l] == 0)) {
-                    current = 1;
-                    currentPos = k;
-                }
-                if ((current > 0) && (histo[l] == 0)) current += 1;
-                if ((current > 0) && (histo[l] != 0)) {
-                    if (current > longest) {
-                        longest = current;
-                        longestPos = currentPos;
-                    }
-                    current = 0;
-                }
-            }
-        } else {
-            longestPos = 0;
-            longest = histo[0];
-            for (int k = 0; k < kModfitBins; k++) {
-                if ((long long)histo[k] < longest) {
-                    longest = histo[k];
-                    longestPos = k;
-                }
-            }
+    if (blk_dih) {
+        const int lane = threadIdx.x & 63;
+        for (int j = threadIdx.x >> 6; j < kFinalizeBlock; j += kFinalizeThreads / 64) {
+            const int g = g0 + j;
+            if (g < dih0 || g >= D) continue;  // the same for the whole wave
+            const int pos = modfit_bin(sh + j * kModfitBins, lane);
+            if (lane == 0) modfit[g] = 2 * pi - (pos + 0.5) * mf_bin;
         }
-        modfit[g] = 2 * pi - (longestPos + 0.5) * mf_bin;
-        return;
     }
+    if (threadIdx.x >= kFinalizeBlock) return;
+    const int g = g0 + threadIdx.x;
+    if (g >= D || g >= dih0) return;
+    const int t = dof_type(nD, g);
     double tmpMin = ord_dec(emin[g]), tmpMax = ord_dec(emax[g]);
     if (t == 0) {
         if ((tmpMin < 0.0) || (tmpMax < 0.0)) atomicOr(flags, kFlagNegBond);
@@ -568,6 +617,16 @@ hipError_t prebin_index_types(const PrepareArgs& a, unsigned g5, int nbx5, int n
                        : prebin_launch<T, true, uint16_t, uint16_t>(a, g5, nbx5, nby5, twopi, s);
 }
 
+// Frames per block of the scans: 2,048 (4 frame lanes of 512), halved down to
+// 128 while the grid would have fewer than 1,024 blocks, so that a short or
+// narrow trajectory still spreads over the CUs instead of leaving a few
+// blocks to walk long frame ranges one load latency after another.
+static int scan_frames_per_block(int nbx, int N) {
+    int fpb = 2048;
+    while (fpb > 128 && (long long)nbx * ((N + fpb - 1) / fpb) < 1024) fpb /= 2;
+    return fpb;
+}
+
 hipError_t launch_prepare(const PrepareArgs& a, hipStream_t s) {
     const int D = a.D, N = a.N, nD = a.nD;
     const int nDih = nD;
@@ -578,21 +637,22 @@ hipError_t launch_prepare(const PrepareArgs& a, hipStream_t s) {
         k_fill_u64<<<64, 256, 0, s>>>(a.emin, (size_t)D, ~0ull);
         k_fill_u64<<<64, 256, 0, s>>>(a.emax, (size_t)D, 0ull);
         if (nDih > 0) k_fill_u32<<<64, 256, 0, s>>>(a.mfhist, (size_t)nDih * kModfitBins, 0u);
-        const int fpb = 2048;
-        const int nbx1 = (D + 63) / 64, nby1 = (N + fpb - 1) / fpb;
+        const int nbx1 = (D + 63) / 64, fpb = scan_frames_per_block(nbx1, N);
+        const int nby1 = (N + fpb - 1) / fpb;
         const unsigned g1 = xcd_grid(nbx1, nby1);
         if (a.double_prec)
             k_scan_raw<double><<<g1, 256, 0, s>>>((const double*)a.values, N, D, nD, fpb, mf_bin, nbx1, nby1, a.emin, a.emax, a.mfhist, a.flags);
         else
             k_scan_raw<float><<<g1, 256, 0, s>>>((const float*)a.values, N, D, nD, fpb, mf_bin, nbx1, nby1, a.emin, a.emax, a.mfhist, a.flags);
-        k_finalize_ranges<<<(D + kFinalizeBlock - 1) / kFinalizeBlock, kFinalizeBlock, 0, s>>>(D, nD, pi, mf_bin, a.emin, a.emax, a.mfhist, a.vmin, a.vmax, a.modfit, a.flags);
+        k_finalize_ranges<<<(D + kFinalizeBlock - 1) / kFinalizeBlock, kFinalizeThreads, 0, s>>>(D, nD, pi, mf_bin, a.emin, a.emax, a.mfhist, a.vmin, a.vmax, a.modfit, a.flags);
         if (nDih > 0) {
-            const int nbx3 = (nDih + 63) / 64, nby3 = (N + fpb - 1) / fpb;
+            const int nbx3 = (nDih + 63) / 64, fpb3 = scan_frames_per_block(nbx3, N);
+            const int nby3 = (N + fpb3 - 1) / fpb3;
             const unsigned g3 = xcd_grid(nbx3, nby3);
             if (a.double_prec)
-                k_scan_dihedral<double><<<g3, 256, 0, s>>>((const double*)a.values, N, D, nD, fpb, twopi, nbx3, nby3, a.modfit, a.emin, a.emax);
+                k_scan_dihedral<double><<<g3, 256, 0, s>>>((const double*)a.values, N, D, nD, fpb3, twopi, nbx3, nby3, a.modfit, a.emin, a.emax);
             else
-                k_scan_dihedral<float><<<g3, 256, 0, s>>>((const float*)a.values, N, D, nD, fpb, twopi, nbx3, nby3, a.modfit, a.emin, a.emax);
+                k_scan_dihedral<float><<<g3, 256, 0, s>>>((const float*)a.values, N, D, nD, fpb3, twopi, nbx3, nby3, a.modfit, a.emin, a.emax);
             k_finalize_dihedral<<<(nDih + 255) / 256, 256, 0, s>>>(D, nD, twopi, a.emin, a.emax, a.vmin, a.vmax);
         }
     }

@@ -15,6 +15,7 @@ import subprocess
 import numpy as np
 import pytest
 
+import modfit_cases as MC
 import oracle_api as O
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -180,6 +181,17 @@ def test_modfit_moves_the_largest_gap_to_the_end():
     x = np.concatenate([x, np.repeat(x[:5 * 100][::5][np.arange(100) != 37], 1)])
     rc, mn, mx, mf, xs = O.dof_range(O.D, x.copy())
     assert mf == two_pi - (37 + 0.5) * binsize
+
+
+@pytest.mark.parametrize("name,counts,expect", MC.CASES, ids=[c[0] for c in MC.CASES])
+def test_modfit_bin_of_crafted_histograms(name, counts, expect):
+    """The oracle's modfit shift on histograms whose bin the reference's two scans
+    (PARENT.cpp:573-607) fix by hand: ties, runs through bins 99 and 0, runs at
+    the 63/64 boundary, a single occupied or a single empty bin."""
+    x = MC.column(counts, O.PI, np.random.default_rng(5))
+    rc, mn, mx, mf, xs = O.dof_range(O.D, x.copy())
+    assert rc == 0
+    assert mf == 2 * O.PI - (expect + 0.5) * MC.bin_width(O.PI)
 
 
 def test_bin_centres_index_exactly():

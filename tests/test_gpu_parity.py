@@ -117,6 +117,35 @@ def test_fp64_trajectory_vs_oracle(engine):
     assert np.max(np.abs(engine.s2d() - s2o)) <= S_ABS
 
 
+def test_modfit_shift_of_crafted_and_random_histograms(engine):
+    """Dihedrals whose 100-bin modfit histograms have hand-checked answers (ties,
+    runs through bins 99 and 0 or across bins 63/64, one occupied or one empty
+    bin, no empty bin) and 200 random histograms: the modfit shift, the ranges
+    and S1D against the oracle, and the crafted ones against their known bin."""
+    import modfit_cases as MC
+    rng = np.random.default_rng(21)
+    cols = [MC.column(c, O.PI, rng) for _, c, _ in MC.CASES]
+    cols += [MC.column(MC.random_counts(rng), O.PI, rng) for _ in range(200)]
+    nD, N = len(cols), MC.FRAMES
+    D = 3 * nD + 3
+    vals = np.empty((N, D), dtype=np.float32)
+    vals[:, :nD + 2] = rng.uniform(1.0, 1.6, (N, nD + 2))
+    vals[:, nD + 2:2 * nD + 3] = rng.uniform(1.5, 2.5, (N, nD + 1))
+    vals[:, 2 * nD + 3:] = np.stack(cols, axis=1)
+    bins = [50] * 6
+    print(f"modfit histograms nD={nD} frames={N}", flush=True)
+    engine.set_trajectory(vals)
+    engine.prepare(bins)
+    s1o, _, mno, mxo, mfo, _ = O.compute(nD, N, vals.T.astype(np.float64), bins)
+    mn, mx, mf = engine.ranges()
+    for j, (name, _, expect) in enumerate(MC.CASES):
+        assert mf[2 * nD + 3 + j] == 2 * O.PI - (expect + 0.5) * MC.bin_width(O.PI), name
+    assert np.array_equal(mf.view(np.uint64), mfo.view(np.uint64))
+    assert np.array_equal(mn.view(np.uint64), mno.view(np.uint64))
+    assert np.array_equal(mx.view(np.uint64), mxo.view(np.uint64))
+    assert np.max(np.abs(engine.s1d() - s1o)) <= 1e-12
+
+
 @pytest.mark.parametrize("method", [0, 1])
 @pytest.mark.parametrize("atoms,frames,seed,bins", CASES[:4])
 def test_counts_bit_exact(engine, method, atoms, frames, seed, bins):
