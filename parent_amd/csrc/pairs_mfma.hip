@@ -371,16 +371,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PERM ? (UPT
             }
             continue;
         }
-        for (int p = wid; p < (dhi - dlo) * TD; p += 4) {
-            const int di = dlo + p / TD, dj = p % TD;
-            const int gi = r0 + di, gj = c0 + dj;
-            if (!(gi < gj && gj < D)) continue;
-            double T = 0.0;
-            // branch-free (the table is 0 at n = 0): the unrolled table loads overlap
+        if constexpr (PERM != 0) {
+            // Up to 12 bins (at most 144 cells a pair, 144 to 4,096 pairs a
+            // tile): one thread per pair, cells in row order, its table loads
+            // in flight together.  A wave per pair left most lanes idle and
+            // waited out one LDS read and one table load per pair.
+            for (int p = tid; p < (dhi - dlo) * TD; p += 256) {
+                const int di = dlo + p / TD, dj = p % TD;
+                const int gi = r0 + di, gj = c0 + dj;
+                if (!(gi < gj && gj < D)) continue;
+                const int* cell = cnt + (di - dlo) * B * kTile + dj * B;
+                double T = 0.0;
+                for (int r = 0; r < B; r++, cell += kTile) {
 #pragma unroll 4
-            for (int c = lane; c < B * B; c += 64) T += in.nlogn[cnt[((di - dlo) * B + c / B) * kTile + dj * B + c % B]];
-            T = wave_total(T);
-            if (lane == 0) in.s2d[par_offset(nD, gi, gj)] = ((in.logN + in.c2[gi]) + in.c2[gj] - 0.5 / N) - T / N;
+                    for (int c = 0; c < B; c++) T += in.nlogn[cell[c]];
+                }
+                in.s2d[par_offset(nD, gi, gj)] = ((in.logN + in.c2[gi]) + in.c2[gj] - 0.5 / N) - T / N;
+            }
+        } else {
+            for (int p = wid; p < (dhi - dlo) * TD; p += 4) {
+                const int di = dlo + p / TD, dj = p % TD;
+                const int gi = r0 + di, gj = c0 + dj;
+                if (!(gi < gj && gj < D)) continue;
+                double T = 0.0;
+                // branch-free (the table is 0 at n = 0): the unrolled table loads overlap
+#pragma unroll 4
+                for (int c = lane; c < B * B; c += 64) T += in.nlogn[cnt[((di - dlo) * B + c / B) * kTile + dj * B + c % B]];
+                T = wave_total(T);
+                if (lane == 0) in.s2d[par_offset(nD, gi, gj)] = ((in.logN + in.c2[gi]) + in.c2[gj] - 0.5 / N) - T / N;
+            }
         }
     }
 }
