@@ -50,9 +50,11 @@ constexpr int kPermMaxBins = 12;  // the register-built operands (PERM) serve B 
 constexpr int kIdxStride = kChunk + 16;  // staged index row stride in bytes (PERM)
 constexpr unsigned kPadIndex = 12;       // staged value of padding frames: v_perm selector 12 gives 0x00
 // PERM: two staging buffers of both sides' index rows (at most 64 dofs a
-// side), then the count tile's row phases in the same memory: at most
-// max over B <= 12 of ceil(64 / B) * B = 72 rows (B = 9 and 12)
+// side; PERM 2, at most 14 dofs a side, adds a remapped copy of both), then
+// the count tile's row phases in the same memory: at most max over B <= 12
+// of ceil(64 / B) * B = 72 rows (B = 9 and 12)
 constexpr int kPermStage = 2 * 2 * 64 * kIdxStride;
+static_assert(2 * (2 * 2 * 14 * kIdxStride) + 64 <= kPermStage, "PERM 2 staging with its remapped copy");
 constexpr int kPermCountRows = 72;
 constexpr int kPermCountBytes = kPermCountRows * kTile * 4;
 constexpr int kPermLds = kPermCountBytes > kPermStage ? kPermCountBytes : kPermStage;
@@ -132,8 +134,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PERM ? (UPT
         // D-1, so the empty padding tiles of a batch queue (r0 = c0 = D)
         // read inside the index matrix too.
         const int side_bytes = TD * kIdxStride, buf_bytes = 2 * side_bytes;
-        // PERM 2: the remapped copy for the rows of bins 8-11, after both buffers
-        const int alt = PERM == 2 ? 2 * buf_bytes : 0;
+        // PERM 2: the remapped copy for the rows of bins 8-11, after both
+        // buffers and 64 bytes further on, so that the copy of a dof sits 16
+        // banks away from the dof's own row: a 32-row operand block reads both
+        // for the same dofs (at an offset that is a multiple of the bank count
+        // the two reads conflicted, 35% of LDS-active cycles at 10 bins).
+        const int alt = PERM == 2 ? 2 * buf_bytes + 64 : 0;
         const int nunits = 2 * TD * 8;
         int ubase[UPT];
         const uint8_t* usrc[UPT];
