@@ -434,9 +434,10 @@ __device__ __forceinline__ void row_histogram(const I* __restrict__ row, int Np,
 }
 
 // The same into 16 lane-sliced copies, h[b * 16 + lane % 16], for bin counts
-// up to 16: the 16 lanes the LDS serves together never share a word, where a
-// single histogram of a few bins serialises most of a wave's atomics (5 bins:
-// 0.98 ms for the 1D pass of 4,494 dofs x 100k frames).
+// up to 64: the 16 lanes the LDS serves together never share a word, where a
+// single histogram of a few dozen bins serialises many of a wave's atomics
+// (the 1D pass of 4,494 dofs x 100k frames: 0.98 ms at 5 bins, 0.39 ms at 20
+// and 0.23 ms at 50 with one histogram; 0.12 ms sliced).
 template <class I>
 __device__ __forceinline__ void row_histogram_sliced(const I* __restrict__ row, int Np, unsigned* h) {
     constexpr int per = 16 / (int)sizeof(I);
@@ -486,11 +487,13 @@ __global__ __launch_bounds__(256) void k_dof_stats(const void* __restrict__ idx2
     const int g = blockIdx.x;
     const int t = dof_type(nD, g);
     const bool sep1 = idx1 != nullptr && B1.of(t) != B2.of(t);
-    // counts of up to 16 bins go through lane-sliced copies, then add up in h2 / h1
-    __shared__ unsigned hs[2][16 * 16];
-    const bool sl2 = nb2 <= 16, sl1 = sep1 && nh1 <= 16;
+    // counts of up to kSliceBins bins go through 16 lane-sliced copies (a
+    // bin's 16 copies in consecutive words), then add up in h2 / h1
+    constexpr int kSliceBins = 64;
+    __shared__ unsigned hs[2][16 * kSliceBins];
+    const bool sl2 = nb2 <= kSliceBins, sl1 = sep1 && nh1 <= kSliceBins;
     for (int i = threadIdx.x; i < nb2 + nh1; i += 256) h2[i] = 0;
-    for (int i = threadIdx.x; i < 2 * 16 * 16; i += 256) hs[i >> 8][i & 255] = 0;
+    for (int i = threadIdx.x; i < 2 * 16 * kSliceBins; i += 256) hs[0][i] = 0;
     __syncthreads();
     if (sl2) row_histogram_sliced(static_cast<const I2*>(idx2) + (size_t)g * Np, Np, hs[0]);
     else row_histogram(static_cast<const I2*>(idx2) + (size_t)g * Np, Np, h2);
@@ -500,11 +503,13 @@ __global__ __launch_bounds__(256) void k_dof_stats(const void* __restrict__ idx2
     }
     if (sl2 || sl1) {
         __syncthreads();
-        const int k = threadIdx.x & 15, side = threadIdx.x >> 4;  // threads 0-31: (bin, histogram)
+        // threads 0-127: (bin, histogram); copy (l + k) % 16 first, so that
+        // the 64 lanes of a read touch 16 different banks, not one
+        const int k = threadIdx.x % kSliceBins, side = threadIdx.x / kSliceBins;
         if (side < 2 && (side ? sl1 && k < nh1 : sl2 && k < nb2)) {
             unsigned n = 0;
 #pragma unroll
-            for (int l = 0; l < 16; l++) n += hs[side][k * 16 + l];
+            for (int l = 0; l < 16; l++) n += hs[side][k * 16 + ((l + k) & 15)];
             (side ? h1 : h2)[k] = n;
         }
     }
