@@ -407,7 +407,8 @@ def run_batch(args, pa, nthreads):
         b.run(method)
         st = b.stats()
         run_ms.append(st["ms_pairs"])
-        log(f"batch pair launch: {run_ms[-1]:.1f} ms (method {st['method']} variant {st['kernel_variant']})")
+        log(f"batch pairs: {run_ms[-1]:.1f} ms in {st['batch_launches']} launch(es) (method {st['method']} "
+            f"variant {st['kernel_variant']} for the most updates)")
     st = b.stats()
     pair_avg = sum(run_ms) / len(run_ms)
     ms_step = prep_ms + pair_avg
@@ -421,7 +422,8 @@ def run_batch(args, pa, nthreads):
     cpu = None if args.no_cpu else cpu_baselines(args.cpu_sample_atoms, 0, 100_000, bins)[0]
     emit(args, updates / (ms_step / 1e3), ms_step,
          {"workload": "batch64_b30", "trajectories": len(shapes), "bins": bins, "updates": updates,
-          "atoms_frames": shapes, "method": st["method"], "variant": st["kernel_variant"]},
+          "atoms_frames": shapes, "method": st["method"], "variant": st["kernel_variant"],
+          "launches": st["batch_launches"]},
          {"phases": {"prepare_ms_total": prep_ms, "pairs_ms": pair_avg,
                      "pairs_kernel_updates_per_s": updates / (pair_avg / 1e3)},
           "mist_totals": mist, "roofline": roofline, "cpu_baseline": cpu, "tuning": tune})

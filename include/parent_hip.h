@@ -156,6 +156,8 @@ typedef struct {
     double ms_sweep;      /* last parent_engine_sweep: the graph launch (prepare + pairs of every set) */
     int sweep_sets;       /* number of bin sets in that sweep */
     int mist_path;        /* last parent_engine_mist: 1 the spanning-tree path, 0 the O(V^2) Prim */
+    int batch_launches;   /* last parent_batch_run: pair kernel launches (tile queues; one per
+                             trajectory on the slab kernel) */
 } parent_stats;
 
 /* One row of the autotuner's table. */
@@ -343,12 +345,24 @@ int parent_batch_add(parent_batch* b, int n_dihedrals, int n_frames, int double_
 int parent_batch_add_binned(parent_batch* b, const int bins[6], int n_dihedrals, int n_frames, int double_prec,
                             const void* values, int on_device, int* index);
 
-/* All pairs of all added trajectories, one kernel launch (method as for
- * parent_engine_pairs; the tile shape is chosen for the batch's largest bin
- * counts;
- * PARENT_METHOD_PLAIN runs the tiled atomic kernel, as the plain kernel has
- * no tile queue).  stats.ms_pairs is that launch. */
+/* All pairs of all added trajectories over merged tile queues (method as
+ * for parent_engine_pairs; the tile shape is chosen for the batch's largest
+ * bin counts; PARENT_METHOD_PLAIN runs the tiled atomic kernel, as the plain
+ * kernel has no tile queue).  One launch, except that PARENT_METHOD_AUTO may
+ * split the batch into two frame-count classes (see
+ * parent_batch_set_classes), each with its own configuration and launch.
+ * stats.ms_pairs covers the launches, stats.batch_launches counts them and
+ * stats.method / kernel_variant name the configuration of the one with the
+ * most updates. */
 int parent_batch_run(parent_batch* b, int method);
+/* Frame-count classes of PARENT_METHOD_AUTO: trajectories of fewer than 20
+ * frames per cell of the batch's largest 2D histogram (18,000 at 30 bins)
+ * against the rest, each class tuned apart and run as its own launch.
+ * mode 1 (default): split when the two classes' predicted times add up to
+ * less than 0.99 of one merged launch's; 2: whenever both classes are
+ * non-empty; 0: never.  The next PARENT_METHOD_AUTO run tunes again.  No
+ * counterpart in the reference (it runs one trajectory per process). */
+int parent_batch_set_classes(parent_batch* b, int mode);
 /* parent_engine_set_variant for the batch's kernels (method
  * PARENT_METHOD_ATOMIC or PARENT_METHOD_SLABS): pins the configuration that
  * parent_batch_run(b, PARENT_METHOD_ATOMIC) uses.  For testing; no
@@ -359,7 +373,9 @@ int parent_batch_set_variant(parent_batch* b, int method, int variant);
  * configuration that fits is timed on the tuning rows of the longest and the
  * shortest trajectory, the two timings fit ms = a * tiles + b * updates, and
  * the configuration with the lowest predicted time for the whole merged
- * queue is kept.  Returns the table (entries[k].ms = predicted ms). */
+ * queue is kept; the same is done for each frame-count class
+ * (parent_batch_set_classes) to decide on a split.  Returns the merged
+ * queue's table (entries[k].ms = predicted ms). */
 int parent_batch_autotune(parent_batch* b, parent_tune_entry* entries, int max_entries, int* n_entries);
 
 /* S1D / S2D of trajectory `index` (either pointer may be NULL). */

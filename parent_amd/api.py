@@ -64,6 +64,7 @@ def lib() -> C.CDLL:
         "parent_engine_sync": (i32, [p]),
         "parent_engine_autotune": (i32, [p, p, i32, C.POINTER(i32)]),
         "parent_batch_set_variant": (i32, [p, i32, i32]),
+        "parent_batch_set_classes": (i32, [p, i32]),
         "parent_engine_set_variant": (i32, [p, i32, i32]),
         "parent_engine_set_mist_path": (i32, [p, i32]),
         "parent_engine_variant_count": (i32, [p, i32, C.POINTER(i32)]),
@@ -106,7 +107,7 @@ class _Stats(C.Structure):
     _fields_ = [("ms_upload", C.c_double), ("ms_prepare", C.c_double), ("ms_pairs", C.c_double),
                 ("ms_mist", C.c_double), ("pair_updates", C.c_double), ("method", C.c_int),
                 ("kernel_variant", C.c_int), ("ms_sweep", C.c_double), ("sweep_sets", C.c_int),
-                ("mist_path", C.c_int)]
+                ("mist_path", C.c_int), ("batch_launches", C.c_int)]
 
 
 def _ptr(a: np.ndarray) -> C.c_void_p:
@@ -474,6 +475,11 @@ class Batch:
     def set_variant(self, method: int, variant: int) -> None:
         """Pins the configuration `run(METHOD_ATOMIC)` uses (-1: default)."""
         self._check(lib().parent_batch_set_variant(self._h, method, variant))
+
+    def set_classes(self, mode: int) -> None:
+        """Frame-count classes of `run(METHOD_AUTO)`: 1 split when predicted
+        faster (default), 2 always when both classes exist, 0 never."""
+        self._check(lib().parent_batch_set_classes(self._h, mode))
 
     def run(self, method: int = METHOD_AUTO) -> None:
         self._check(lib().parent_batch_run(self._h, method))

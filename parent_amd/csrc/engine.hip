@@ -155,11 +155,26 @@ struct parent_batch {
         BinSet set;
     };
     std::vector<std::unique_ptr<Item>> items;
-    DevBuf queue, descs;
-    int queue_items = -1, queue_method = -1, queue_variant = -1, ntiles = 0;
-    const void* queue_nlogn = nullptr;  // the n ln n table the uploaded descriptors point to
+    // One kernel launch of a run: some of the items (indices into items) and
+    // the configuration they run with.
+    struct Plan {
+        std::vector<int> members;
+        int method = -1, variant = -1;
+    };
+    // A launch's merged tile queue on the device, built for one Plan.
+    struct Queue {
+        Plan plan;
+        DevBuf tiles;
+        int ntiles = 0;
+    };
+    std::vector<std::unique_ptr<Queue>> queues;  // the last run's launches
+    DevBuf descs;                                // PairInputs of every item
+    int descs_items = -1;
+    const void* descs_nlogn = nullptr;  // the n ln n table the uploaded descriptors point to
     int method = PARENT_METHOD_PLAIN, variant = 0;
-    int tuned_items = -1, tuned_method = -1, tuned_variant = -1;  // batch autotuner's choice
+    int classes = 1;                      // parent_batch_set_classes
+    int tuned_items = -1;                 // items when tuned_plan was made
+    std::vector<Plan> tuned_plan;         // the batch autotuner's launches
 };
 
 namespace {
@@ -776,17 +791,18 @@ BinSet batch_envelope(const parent_batch* b) {
     return v;
 }
 
-// Kernel choice for a whole batch.  One configuration runs every tile of the
-// merged queue, and the right one depends on the mix: per pair, a tile costs
-// a fixed part (clearing its LDS histograms, the epilogue) and a part per
-// frame.  Each candidate is timed on a tuning sample (tune_sample: 8 runs of
-// rows) of the batch's longest and shortest trajectories; the two timings
-// give the model
-// ms = alpha * tiles + beta * updates, which predicts the full queue.
-int batch_autotune(parent_engine* e, parent_batch* b, int* method, int* variant) {
-    const int n = (int)b->items.size();
-    int kl = 0, ks = 0;
-    for (int k = 1; k < n; k++) {
+// Kernel choice for a set of a batch's items (`members`) that share one
+// launch.  One configuration runs every tile of their merged queue, and the
+// right one depends on the mix: per pair, a tile costs a fixed part
+// (clearing its LDS histograms, the epilogue) and a part per frame.  Each
+// candidate is timed on a tuning sample (tune_sample: 8 runs of rows) of the
+// members' longest and shortest trajectories; the two timings give the model
+// ms = alpha * tiles + beta * updates, which predicts the full queue
+// (*pred_ms for the chosen configuration).
+int batch_autotune(parent_engine* e, parent_batch* b, const std::vector<int>& members, int* method, int* variant,
+                   double* pred_ms) {
+    int kl = members[0], ks = members[0];
+    for (int k : members) {
         const Traj& t = b->items[k]->traj;
         if (t.N > b->items[kl]->traj.N || (t.N == b->items[kl]->traj.N && t.D > b->items[kl]->traj.D)) kl = k;
         if (t.N < b->items[ks]->traj.N || (t.N == b->items[ks]->traj.N && t.D > b->items[ks]->traj.D)) ks = k;
@@ -798,7 +814,7 @@ int batch_autotune(parent_engine* e, parent_batch* b, int* method, int* variant)
     if (mfma_fits(e, s0)) cand.push_back({PARENT_METHOD_MFMA, 0});
     if (cand.empty()) return bfail(b, PARENT_E_UNSUPPORTED, "no tiled kernel fits these bins");
     double total_upd = 0;
-    for (auto& it : b->items) total_upd += (double)n_pairs(it->traj.nD) * it->traj.N;
+    for (int k : members) total_upd += (double)n_pairs(b->items[k]->traj.nD) * b->items[k]->traj.N;
     e->last_tune.clear();
     DevBuf dt;
     double best = -1;
@@ -841,7 +857,8 @@ int batch_autotune(parent_engine* e, parent_batch* b, int* method, int* variant)
             }
         }
         double total_tiles = 0;
-        for (auto& it : b->items) total_tiles += (double)count_tiles(it->traj.D, it->traj.Np, tr, tc);
+        for (int k : members)
+            total_tiles += (double)count_tiles(b->items[k]->traj.D, b->items[k]->traj.Np, tr, tc);
         const double pred = alpha * total_tiles + beta * total_upd;
         const int block = m == PARENT_METHOD_MFMA ? 256 : atomic_config(id).block;
         e->last_tune.push_back(parent_tune_entry{m, id, tr, tc, block, pred, total_upd, total_upd / (pred * 1e-3)});
@@ -851,6 +868,92 @@ int batch_autotune(parent_engine* e, parent_batch* b, int* method, int* variant)
             *variant = id;
         }
     }
+    *pred_ms = best;
+    return PARENT_OK;
+}
+
+// Trajectories with fewer frames than this many per cell of the batch's
+// largest 2D histogram form the short class (18,000 frames at 30 bins).
+constexpr int kShortFramesPerCell = 20;
+
+// The launches PARENT_METHOD_AUTO runs.  One launch of the whole merged
+// queue, or, when the batch holds short trajectories and longer ones, one
+// launch per class, each tuned on its own: a short trajectory's tiles spend
+// much of their time on the fixed part (clear, epilogue), and their counts
+// stay small enough for narrower counters, so the two classes favour
+// different configurations.  The split is kept when the two classes'
+// predicted times add up to less than 0.99 of the merged queue's (classes 1),
+// always (2) or never (0).  e->last_tune keeps the merged queue's table.
+int plan_batch(parent_engine* e, parent_batch* b) {
+    const int n = (int)b->items.size();
+    std::vector<int> all(n);
+    std::iota(all.begin(), all.end(), 0);
+    parent_batch::Plan merged;
+    merged.members = all;
+    double p_all = 0;
+    if (int rc = batch_autotune(e, b, all, &merged.method, &merged.variant, &p_all)) return rc;
+    const std::vector<parent_tune_entry> table = e->last_tune;
+    std::vector<parent_batch::Plan> plan{merged};
+    const double bm = bmax2d(batch_envelope(b));
+    parent_batch::Plan lng, shrt;
+    for (int k = 0; k < n; k++)
+        (b->items[k]->traj.N < kShortFramesPerCell * bm * bm ? shrt : lng).members.push_back(k);
+    if (b->classes != 0 && !lng.members.empty() && !shrt.members.empty()) {
+        double p_long = 0, p_short = 0;
+        if (int rc = batch_autotune(e, b, lng.members, &lng.method, &lng.variant, &p_long)) return rc;
+        if (int rc = batch_autotune(e, b, shrt.members, &shrt.method, &shrt.variant, &p_short)) return rc;
+        if (b->classes == 2 || p_long + p_short < 0.99 * p_all) plan = {lng, shrt};
+    }
+    e->last_tune = table;
+    b->tuned_plan = plan;
+    b->tuned_items = n;
+    return PARENT_OK;
+}
+
+// Uploads the merged tile queue of one launch: its items by decreasing frame
+// count (cost per tile), each item's tiles in its band order, padded with
+// empty tiles to a multiple of 8 and dealt so that block b of a launch runs
+// entry b of this list whatever XCD it lands on (the kernels map block b to
+// list slot (b % 8) * chunk + b / 8).
+int build_queue(parent_engine* e, parent_batch* b, const BinSet& env, parent_batch::Queue& qu) {
+    const parent_batch::Plan& pl = qu.plan;
+    int tr, tc;
+    tile_shape(env, pl.method, pl.variant, &tr, &tc);
+    std::vector<int> order = pl.members;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return b->items[x]->traj.N > b->items[y]->traj.N; });
+    std::vector<int> lst;
+    for (int k : order) {
+        const Traj& t = b->items[k]->traj;
+        std::vector<int> tl = make_tiles(t.D, t.Np, tr, tc);
+        for (size_t i = 0; i + 1 < tl.size(); i += 2) {
+            lst.push_back(tl[i]);
+            lst.push_back(tl[i + 1]);
+            lst.push_back(k);
+        }
+    }
+    while ((lst.size() / 3) % 8) {
+        const Traj& t0 = b->items[order[0]]->traj;
+        lst.push_back(t0.D);  // no valid pair: r0 = c0 = D
+        lst.push_back(t0.D);
+        lst.push_back(order[0]);
+    }
+    if (lst.size() / 3 > (size_t)INT32_MAX) return bfail(b, PARENT_E_UNSUPPORTED, "batch tile queue too long");
+    const int T = (int)(lst.size() / 3);
+    // the launchers issue the queue in launches of `seg` tiles (a multiple
+    // of 8), each with its own block-to-slot map: deal each segment alone
+    const int seg = pl.method == PARENT_METHOD_MFMA ? mfma_tiles_per_launch() : atomic_tiles_per_launch(pl.variant);
+    std::vector<int> q(lst.size());
+    for (int s0 = 0; s0 < T; s0 += seg) {
+        const int n_seg = std::min(seg, T - s0), chunk = n_seg / 8;
+        for (int blk = 0; blk < n_seg; blk++) {
+            const int slot = s0 + (blk % 8) * chunk + blk / 8;
+            for (int c = 0; c < 3; c++) q[3 * (size_t)slot + c] = lst[3 * ((size_t)s0 + blk) + c];
+        }
+    }
+    HIPCHK(e, qu.tiles.ensure(q.size() * 4));
+    HIPCHK(e, hipMemcpyAsync(qu.tiles.p, q.data(), q.size() * 4, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    qu.ntiles = T;
     return PARENT_OK;
 }
 
@@ -1319,10 +1422,18 @@ int parent_engine_set_variant(parent_engine* e, int method, int variant) {
     return PARENT_OK;
 }
 
+int parent_batch_set_classes(parent_batch* b, int mode) {
+    if (!b) return PARENT_E_INVALID;
+    if (mode < 0 || mode > 2) return bfail(b, PARENT_E_INVALID, "classes mode must be 0, 1 or 2");
+    b->classes = mode;
+    b->tuned_items = -1;  // plan again on the next PARENT_METHOD_AUTO run
+    return PARENT_OK;
+}
+
 int parent_batch_set_variant(parent_batch* b, int method, int variant) {
     if (!b) return PARENT_E_INVALID;
     const int rc = parent_engine_set_variant(b->eng, method, variant);
-    b->queue_items = -1;  // the merged queue is dealt for the kernel's launch size
+    b->queues.clear();  // a merged queue is dealt for its kernel's launch size
     return rc;
 }
 
@@ -1382,7 +1493,7 @@ void parent_batch_destroy(parent_batch* b) {
         (void)hipStreamSynchronize(b->eng->stream);
     }
     b->items.clear();
-    b->queue.release();
+    b->queues.clear();
     b->descs.release();
     parent_engine_destroy(b->eng);
     delete b;
@@ -1422,8 +1533,10 @@ int parent_batch_add_binned(parent_batch* b, const int bins[6], int n_dihedrals,
     memcpy(it->set.bins, bins, sizeof(it->set.bins));
     // run_prepare may reallocate the shared n ln n table (a longer
     // trajectory), which the uploaded queue descriptors point to: drop the
-    // queue before, so that a rejected trajectory leaves no stale pointer
-    b->queue_items = -1;
+    // queues and descriptors before, so that a rejected trajectory leaves no
+    // stale pointer
+    b->queues.clear();
+    b->descs_items = -1;
     const int rc = run_prepare(e, t, it->set);
     // the index matrix and per-dof constants are all the pairs need: the
     // uploaded values go right away
@@ -1441,8 +1554,8 @@ int parent_batch_run(parent_batch* b, int method) {
     if (b->items.empty()) return bfail(b, PARENT_E_INVALID, "empty batch");
     HIPCHK(e, hipSetDevice(e->device));
     const int n = (int)b->items.size();
-    // kernel choice on the item with the most pair-frame updates (one tile
-    // shape for the whole queue; the bins are common to all items)
+    // the item with the most pair-frame updates: the PairInputs argument of
+    // the launches (the batch instantiations read each tile's own descriptor)
     int big = 0;
     for (int k = 1; k < n; k++) {
         const Traj &a = b->items[k]->traj, &c = b->items[big]->traj;
@@ -1456,7 +1569,6 @@ int parent_batch_run(parent_batch* b, int method) {
     BinSet& sb = b->items[big]->set;
     const Traj& tb = b->items[big]->traj;
     const BinSet env = batch_envelope(b);  // the one kernel must hold every trajectory's bins
-    int used = method, variant = 0;
     if (method < PARENT_METHOD_PLAIN || method > PARENT_METHOD_SLABS) return bfail(b, PARENT_E_INVALID, "unknown method");
     if (method == PARENT_METHOD_SLABS || needs_slabs(e, env)) {
         // bins the LDS kernels cannot hold: the slab kernel, one launch per
@@ -1481,99 +1593,84 @@ int parent_batch_run(parent_batch* b, int method) {
         e->stats.pair_updates = upd;
         e->stats.method = PARENT_METHOD_SLABS;
         e->stats.kernel_variant = e->forced_slab_rows;
+        e->stats.batch_launches = (int)b->items.size();
         return PARENT_OK;
     }
+    std::vector<parent_batch::Plan> plan;
     if (method == PARENT_METHOD_AUTO) {
-        if (b->tuned_items != n) {
-            if (int rc = batch_autotune(e, b, &b->tuned_method, &b->tuned_variant)) return rc;
-            b->tuned_items = n;
-        }
-        used = b->tuned_method;
-        variant = b->tuned_variant;
-    } else if (method == PARENT_METHOD_MFMA && mfma_fits(e, env)) {
-        used = PARENT_METHOD_MFMA;
-        variant = 0;
+        if (b->tuned_items != n)
+            if (int rc = plan_batch(e, b)) return rc;
+        plan = b->tuned_plan;
     } else {
-        // ATOMIC, MFMA outside its bin range, and PLAIN (which has no tile
-        // queue): the tiled atomic kernel, in a configuration whose counters
-        // hold the longest trajectory
-        used = PARENT_METHOD_ATOMIC;
-        variant = choose_atomic_config(e, env, nmax);
-        if (variant < 0) return bfail(b, PARENT_E_UNSUPPORTED, "no tiled kernel fits these bins");
+        parent_batch::Plan one;
+        one.members.resize(n);
+        std::iota(one.members.begin(), one.members.end(), 0);
+        if (method == PARENT_METHOD_MFMA && mfma_fits(e, env)) {
+            one.method = PARENT_METHOD_MFMA;
+            one.variant = 0;
+        } else {
+            // ATOMIC, MFMA outside its bin range, and PLAIN (which has no tile
+            // queue): the tiled atomic kernel, in a configuration whose counters
+            // hold the longest trajectory
+            one.method = PARENT_METHOD_ATOMIC;
+            one.variant = choose_atomic_config(e, env, nmax);
+            if (one.variant < 0) return bfail(b, PARENT_E_UNSUPPORTED, "no tiled kernel fits these bins");
+        }
+        plan = {one};
     }
-    int tr, tc;
-    tile_shape(env, used, variant, &tr, &tc);
-    if (b->queue_items != n || b->queue_method != used || b->queue_variant != variant ||
-        b->queue_nlogn != e->scratch.nlogn.p) {
-        // Merged queue: items by decreasing frame count (cost per tile), each
-        // item's tiles in its band order, padded with empty tiles to a
-        // multiple of 8 and dealt so that block b of the launch runs entry b
-        // of this list whatever XCD it lands on (the kernels map block b to
-        // list slot (b % 8) * chunk + b / 8).
-        std::vector<int> order(n);
-        std::iota(order.begin(), order.end(), 0);
-        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return b->items[x]->traj.N > b->items[y]->traj.N; });
-        std::vector<int> lst;
-        for (int k : order) {
-            const Traj& t = b->items[k]->traj;
-            std::vector<int> tl = make_tiles(t.D, t.Np, tr, tc);
-            for (size_t i = 0; i + 1 < tl.size(); i += 2) {
-                lst.push_back(tl[i]);
-                lst.push_back(tl[i + 1]);
-                lst.push_back(k);
-            }
-        }
-        while ((lst.size() / 3) % 8) {
-            const Traj& t0 = b->items[order[0]]->traj;
-            lst.push_back(t0.D);  // no valid pair: r0 = c0 = D
-            lst.push_back(t0.D);
-            lst.push_back(order[0]);
-        }
-        if (lst.size() / 3 > (size_t)INT32_MAX) return bfail(b, PARENT_E_UNSUPPORTED, "batch tile queue too long");
-        const int T = (int)(lst.size() / 3);
-        // the launchers issue the queue in launches of `seg` tiles (a multiple
-        // of 8), each with its own block-to-slot map: deal each segment alone
-        const int seg = used == PARENT_METHOD_MFMA ? mfma_tiles_per_launch() : atomic_tiles_per_launch(variant);
-        std::vector<int> q(lst.size());
-        for (int s0 = 0; s0 < T; s0 += seg) {
-            const int n_seg = std::min(seg, T - s0), chunk = n_seg / 8;
-            for (int blk = 0; blk < n_seg; blk++) {
-                const int slot = s0 + (blk % 8) * chunk + blk / 8;
-                for (int c = 0; c < 3; c++) q[3 * (size_t)slot + c] = lst[3 * ((size_t)s0 + blk) + c];
-            }
-        }
+    if (b->descs_items != n || b->descs_nlogn != e->scratch.nlogn.p) {
         std::vector<PairInputs> d(n);
         for (int k = 0; k < n; k++) d[k] = with_nlogn(e, pair_inputs(b->items[k]->traj, b->items[k]->set));
-        HIPCHK(e, b->queue.ensure(q.size() * 4));
         HIPCHK(e, b->descs.ensure(d.size() * sizeof(PairInputs)));
-        HIPCHK(e, hipMemcpyAsync(b->queue.p, q.data(), q.size() * 4, hipMemcpyHostToDevice, e->stream));
         HIPCHK(e, hipMemcpyAsync(b->descs.p, d.data(), d.size() * sizeof(PairInputs), hipMemcpyHostToDevice, e->stream));
         HIPCHK(e, hipStreamSynchronize(e->stream));
-        b->queue_items = n;
-        b->queue_nlogn = e->scratch.nlogn.p;
-        b->queue_method = used;
-        b->queue_variant = variant;
-        b->ntiles = T;
+        b->descs_items = n;
+        b->descs_nlogn = e->scratch.nlogn.p;
+    }
+    if (b->queues.size() != plan.size()) {
+        b->queues.clear();
+        for (size_t i = 0; i < plan.size(); i++) b->queues.emplace_back(new parent_batch::Queue());
+    }
+    for (size_t i = 0; i < plan.size(); i++) {
+        parent_batch::Queue& qu = *b->queues[i];
+        if (qu.ntiles > 0 && qu.plan.members == plan[i].members && qu.plan.method == plan[i].method &&
+            qu.plan.variant == plan[i].variant)
+            continue;
+        qu.plan = plan[i];
+        qu.ntiles = 0;
+        if (int rc = build_queue(e, b, env, qu)) return rc;
     }
     const PairInputs in0 = with_nlogn(e, pair_inputs(tb, sb));  // unused by the batch instantiations
-    const TileQueue q{b->queue.as<int>(), b->ntiles, 3, b->descs.as<PairInputs>()};
     HIPCHK(e, hipEventRecord(e->ev[4], e->stream));
-    HIPCHK(e, launch_tiled(e, in0, bmax2d(env), used, variant, q, nullptr, nullptr, nullptr));
+    for (auto& qu : b->queues) {
+        const TileQueue q{qu->tiles.as<int>(), qu->ntiles, 3, b->descs.as<PairInputs>()};
+        HIPCHK(e, launch_tiled(e, in0, bmax2d(env), qu->plan.method, qu->plan.variant, q, nullptr, nullptr, nullptr));
+    }
     HIPCHK(e, hipEventRecord(e->ev[5], e->stream));
     HIPCHK(e, hipStreamSynchronize(e->stream));
-    double upd = 0;
-    for (auto& it : b->items) {
-        it->set.s2d_valid = true;
-        it->set.method = used;
-        it->set.variant = variant;
-        upd += (double)n_pairs(it->traj.nD) * it->traj.N;
+    // stats name the configuration of the launch with the most updates
+    double upd = 0, top = -1;
+    for (auto& qu : b->queues) {
+        double u = 0;
+        for (int k : qu->plan.members) {
+            BinSet& s = b->items[k]->set;
+            s.s2d_valid = true;
+            s.method = qu->plan.method;
+            s.variant = qu->plan.variant;
+            u += (double)n_pairs(b->items[k]->traj.nD) * b->items[k]->traj.N;
+        }
+        upd += u;
+        if (u > top) {
+            top = u;
+            b->method = qu->plan.method;
+            b->variant = qu->plan.variant;
+        }
     }
-    b->method = used;
-    b->variant = variant;
     e->stats.ms_pairs = elapsed(e->ev[4], e->ev[5]);
     e->stats.pair_updates = upd;
-    e->stats.method = used;
-    e->stats.kernel_variant = variant;
+    e->stats.method = b->method;
+    e->stats.kernel_variant = b->variant;
+    e->stats.batch_launches = (int)b->queues.size();
     return PARENT_OK;
 }
 
@@ -1586,8 +1683,7 @@ int parent_batch_autotune(parent_batch* b, parent_tune_entry* entries, int max_e
     for (auto& it : b->items) nmax = std::max(nmax, it->traj.N);
     HIPCHK(e, e->scratch.nlogn.ensure(((size_t)nmax + 1) * 8));
     HIPCHK(e, launch_nlogn(nmax, e->scratch.nlogn.as<double>(), e->stream));
-    if (int rc = batch_autotune(e, b, &b->tuned_method, &b->tuned_variant)) return rc;
-    b->tuned_items = (int)b->items.size();
+    if (int rc = plan_batch(e, b)) return rc;
     const int n = (int)e->last_tune.size();
     if (n_entries) *n_entries = n;
     for (int k = 0; k < n && k < max_entries && entries; k++) entries[k] = e->last_tune[k];

@@ -131,6 +131,39 @@ def test_batch_of_uneven_trajectories(bins, method):
             assert np.max(np.abs(tot - tot1)) <= 1e-8
 
 
+def test_batch_frame_classes():
+    """PARENT_METHOD_AUTO on a batch of short (under 18,000 frames at 30 bins)
+    and long trajectories: forced into two class launches, forced into one,
+    and left to the autotuner, every trajectory's S1D bit for bit and S2D
+    and MI against the oracle, and the two-launch S2D within 1e-10 of the
+    one-launch S2D."""
+    pa = _pa()
+    bins = [30] * 6
+    shapes = [(12, 20000, 1), (9, 24001, 2), (15, 3000, 3), (8, 900, 4), (20, 6000, 5)]
+    trajs = [pa.synth_values(a, n, s) for a, n, s in shapes]
+    with pa.Batch(bins) as b:
+        for v in trajs:
+            b.add(v)
+        runs = {}
+        for mode, launches in ((2, 2), (0, 1), (1, None)):
+            print(f"frame classes: mode {mode}", flush=True)
+            b.set_classes(mode)
+            b.run(pa.METHOD_AUTO)
+            st = b.stats()
+            if launches is not None:
+                assert st["batch_launches"] == launches
+            assert st["pair_updates"] == sum((3 * a - 6) * (3 * a - 7) // 2 * n for a, n, _ in shapes)
+            runs[mode] = [b.results(k) for k in range(len(trajs))]
+        for k, v in enumerate(trajs):
+            nD = v.shape[1] // 3 - 1
+            (s1a, s2a), (s1b, s2b), (s1c, s2c) = runs[2][k], runs[0][k], runs[1][k]
+            assert np.array_equal(s1a.view(np.uint64), s1b.view(np.uint64))
+            assert np.array_equal(s1a.view(np.uint64), s1c.view(np.uint64))
+            assert np.max(np.abs(s2a - s2b)) <= S_ABS
+            _check_vs_oracle(nD, v, bins, s1a, s2a)
+            _check_vs_oracle(nD, v, bins, s1c, s2c)
+
+
 @pytest.mark.parametrize("method", [1, 2, 3])  # atomic, MFMA, auto
 def test_batch_with_short_trajectories(method):
     """Trajectories of 1 to 17 frames beside longer ones in one launch: the
