@@ -358,9 +358,9 @@ int parent_batch_run(parent_batch* b, int method);
 /* Frame-count classes of PARENT_METHOD_AUTO: trajectories of fewer than 20
  * frames per cell of the batch's largest 2D histogram (18,000 at 30 bins)
  * against the rest, each class tuned apart and run as its own launch.
- * mode 1 (default): split when the two classes' predicted times add up to
- * less than 0.99 of one merged launch's; 2: whenever both classes are
- * non-empty; 0: never.  The next PARENT_METHOD_AUTO run tunes again.  No
+ * mode 1 (default): split when the two classes' estimated times add up to
+ * less than one merged launch's; 2: whenever both classes are non-empty;
+ * 0: never.  The next PARENT_METHOD_AUTO run tunes again.  No
  * counterpart in the reference (it runs one trajectory per process). */
 int parent_batch_set_classes(parent_batch* b, int mode);
 /* parent_engine_set_variant for the batch's kernels (method
@@ -371,11 +371,13 @@ int parent_batch_set_variant(parent_batch* b, int method, int variant);
 
 /* The batch autotuner (what PARENT_METHOD_AUTO runs on first use): every
  * configuration that fits is timed on the tuning rows of the longest and the
- * shortest trajectory, the two timings fit ms = a * tiles + b * updates, and
- * the configuration with the lowest predicted time for the whole merged
- * queue is kept; the same is done for each frame-count class
+ * shortest trajectory, the two timings fit ms = a * tiles + b * updates for
+ * the whole merged queue, the 3 lowest predictions are timed again on a
+ * sample of that queue (64 contiguous runs, 1/64 of its tiles), and the
+ * fastest there is kept.  The same is done for each frame-count class
  * (parent_batch_set_classes) to decide on a split.  Returns the merged
- * queue's table (entries[k].ms = predicted ms). */
+ * queue's table (entries[k].ms = predicted ms; for the 3 finalists, the
+ * estimate from the queue sample). */
 int parent_batch_autotune(parent_batch* b, parent_tune_entry* entries, int max_entries, int* n_entries);
 
 /* S1D / S2D of trajectory `index` (either pointer may be NULL). */

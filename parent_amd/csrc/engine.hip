@@ -791,14 +791,152 @@ BinSet batch_envelope(const parent_batch* b) {
     return v;
 }
 
+// A launch's list of (row, column, item) tiles, padded with empty tiles to a
+// multiple of 8 and dealt so that block b of a launch runs entry b of the
+// list whatever XCD it lands on (the kernels map block b to list slot
+// (b % 8) * chunk + b / 8).  The launchers issue a queue in launches of `seg`
+// tiles (a multiple of 8), each with its own block-to-slot map, so each
+// segment is dealt alone.  (pad_D, pad_item): a trajectory whose r0 = c0 = D
+// tile holds no pair.
+std::vector<int> deal_queue(std::vector<int> lst, int pad_D, int pad_item, int seg) {
+    while ((lst.size() / 3) % 8) {
+        lst.push_back(pad_D);
+        lst.push_back(pad_D);
+        lst.push_back(pad_item);
+    }
+    const long long T = (long long)(lst.size() / 3);
+    std::vector<int> q(lst.size());
+    for (long long s0 = 0; s0 < T; s0 += seg) {
+        const long long n_seg = std::min<long long>(seg, T - s0), chunk = n_seg / 8;
+        for (long long blk = 0; blk < n_seg; blk++) {
+            const long long slot = s0 + (blk % 8) * chunk + blk / 8;
+            for (int c = 0; c < 3; c++) q[3 * (size_t)slot + c] = lst[3 * ((size_t)s0 + blk) + c];
+        }
+    }
+    return q;
+}
+
+int queue_segment(int method, int variant) {
+    return method == PARENT_METHOD_MFMA ? mfma_tiles_per_launch() : atomic_tiles_per_launch(variant);
+}
+
+std::vector<int> by_frames_desc(const parent_batch* b, std::vector<int> members) {
+    std::stable_sort(members.begin(), members.end(),
+                     [&](int x, int y) { return b->items[x]->traj.N > b->items[y]->traj.N; });
+    return members;
+}
+
+// Uploads the PairInputs of every item (each tile's descriptor).
+int ensure_descs(parent_engine* e, parent_batch* b) {
+    const int n = (int)b->items.size();
+    if (b->descs_items == n && b->descs_nlogn == e->scratch.nlogn.p) return PARENT_OK;
+    std::vector<PairInputs> d(n);
+    for (int k = 0; k < n; k++) d[k] = with_nlogn(e, pair_inputs(b->items[k]->traj, b->items[k]->set));
+    HIPCHK(e, b->descs.ensure(d.size() * sizeof(PairInputs)));
+    HIPCHK(e, hipMemcpyAsync(b->descs.p, d.data(), d.size() * sizeof(PairInputs), hipMemcpyHostToDevice, e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    b->descs_items = n;
+    b->descs_nlogn = e->scratch.nlogn.p;
+    return PARENT_OK;
+}
+
+// Uploads the merged tile queue of one launch: its items by decreasing frame
+// count (cost per tile), each item's tiles in its band order.
+int build_queue(parent_engine* e, parent_batch* b, const BinSet& env, parent_batch::Queue& qu) {
+    const parent_batch::Plan& pl = qu.plan;
+    int tr, tc;
+    tile_shape(env, pl.method, pl.variant, &tr, &tc);
+    const std::vector<int> order = by_frames_desc(b, pl.members);
+    std::vector<int> lst;
+    for (int k : order) {
+        const Traj& t = b->items[k]->traj;
+        std::vector<int> tl = make_tiles(t.D, t.Np, tr, tc);
+        for (size_t i = 0; i + 1 < tl.size(); i += 2) {
+            lst.push_back(tl[i]);
+            lst.push_back(tl[i + 1]);
+            lst.push_back(k);
+        }
+    }
+    if (lst.size() / 3 + 7 > (size_t)INT32_MAX) return bfail(b, PARENT_E_UNSUPPORTED, "batch tile queue too long");
+    const std::vector<int> q = deal_queue(std::move(lst), b->items[order[0]]->traj.D, order[0],
+                                          queue_segment(pl.method, pl.variant));
+    HIPCHK(e, qu.tiles.ensure(q.size() * 4));
+    HIPCHK(e, hipMemcpyAsync(qu.tiles.p, q.data(), q.size() * 4, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    qu.ntiles = (int)(q.size() / 3);
+    return PARENT_OK;
+}
+
+// Times configuration (m, id) on a sample of the merged queue of `members`
+// and returns the full queue's estimated time: kBatchRuns contiguous runs
+// spread evenly over the queue (contiguous, so that the tiles of a run share
+// index rows in L2 as in the full launch), 1/64 of its tiles and at least
+// kFinalTilesPerCu per compute unit (the whole queue when it is smaller).
+constexpr int kBatchRuns = 64;
+int time_queue_sample(parent_engine* e, parent_batch* b, const std::vector<int>& members, int m, int id,
+                      double* est_ms) {
+    const BinSet env = batch_envelope(b);
+    int tr, tc;
+    tile_shape(env, m, id, &tr, &tc);
+    const std::vector<int> order = by_frames_desc(b, members);
+    std::vector<long long> cnt(order.size());
+    long long total = 0;
+    for (size_t i = 0; i < order.size(); i++) {
+        const Traj& t = b->items[order[i]]->traj;
+        cnt[i] = count_tiles(t.D, t.Np, tr, tc);
+        total += cnt[i];
+    }
+    if (total < 1) return bfail(b, PARENT_E_INVALID, "batch has no pairs");
+    const long long want = std::min(total, std::max((long long)kFinalTilesPerCu * e->cus, total / 64));
+    const long long R = std::min<long long>(kBatchRuns, want), L = want / R;
+    std::vector<int> lst;
+    long long off = 0;
+    for (size_t i = 0; i < order.size(); i++) {
+        const Traj& t = b->items[order[i]]->traj;
+        const long long a = off, z = off + cnt[i];
+        std::vector<int> tl;  // the item's tiles, listed when a run reaches it
+        for (long long r = 0; r < R; r++) {
+            const long long s0 = std::max(0LL, std::min((2 * r + 1) * total / (2 * R) - L / 2, total - L));
+            const long long lo = std::max(s0, a), hi = std::min(s0 + L, z);
+            if (lo >= hi) continue;
+            if (tl.empty()) tl = make_tiles(t.D, t.Np, tr, tc);
+            for (long long x = lo; x < hi; x++) {
+                lst.push_back(tl[2 * (x - a)]);
+                lst.push_back(tl[2 * (x - a) + 1]);
+                lst.push_back(order[i]);
+            }
+        }
+        off = z;
+    }
+    const double sampled = (double)(lst.size() / 3);
+    const std::vector<int> q = deal_queue(std::move(lst), b->items[order[0]]->traj.D, order[0], queue_segment(m, id));
+    DevBuf dq;
+    HIPCHK(e, dq.ensure(q.size() * 4));
+    HIPCHK(e, hipMemcpyAsync(dq.p, q.data(), q.size() * 4, hipMemcpyHostToDevice, e->stream));
+    if (int rc = ensure_descs(e, b)) return rc;
+    const TileQueue tq{dq.as<int>(), (int)(q.size() / 3), 3, b->descs.as<PairInputs>()};
+    const Traj& t0 = b->items[order[0]]->traj;
+    const PairInputs in0 = with_nlogn(e, pair_inputs(t0, b->items[order[0]]->set));  // unused by the batch kernels
+    double ms = 0;
+    if (int rc = time_launch(e, [&] { return launch_tiled(e, in0, bmax2d(env), m, id, tq, nullptr, nullptr, nullptr); }, &ms))
+        return rc;
+    *est_ms = ms * (double)total / sampled;
+    return PARENT_OK;
+}
+
 // Kernel choice for a set of a batch's items (`members`) that share one
 // launch.  One configuration runs every tile of their merged queue, and the
 // right one depends on the mix: per pair, a tile costs a fixed part
 // (clearing its LDS histograms, the epilogue) and a part per frame.  Each
 // candidate is timed on a tuning sample (tune_sample: 8 runs of rows) of the
 // members' longest and shortest trajectories; the two timings give the model
-// ms = alpha * tiles + beta * updates, which predicts the full queue
-// (*pred_ms for the chosen configuration).
+// ms = alpha * tiles + beta * updates, which predicts the full queue.  That
+// model ranks the candidates only roughly (two configurations 1% apart in
+// prediction ran 880 and 1,060 ms on the long class of a 64-trajectory
+// batch), so the kBatchFinalists best are then timed on a sample of the
+// merged queue itself (time_queue_sample), and the fastest there is kept
+// (*pred_ms: its estimate for the full queue).
+constexpr int kBatchFinalists = 3;
 int batch_autotune(parent_engine* e, parent_batch* b, const std::vector<int>& members, int* method, int* variant,
                    double* pred_ms) {
     int kl = members[0], ks = members[0];
@@ -868,6 +1006,24 @@ int batch_autotune(parent_engine* e, parent_batch* b, const std::vector<int>& me
             *variant = id;
         }
     }
+    // second stage: the kBatchFinalists best predictions timed on a sample
+    // of the members' own merged queue; their table rows take that estimate
+    std::vector<size_t> rank(e->last_tune.size());
+    std::iota(rank.begin(), rank.end(), 0);
+    std::stable_sort(rank.begin(), rank.end(), [&](size_t x, size_t y) { return e->last_tune[x].ms < e->last_tune[y].ms; });
+    best = -1;
+    for (size_t f = 0; f < rank.size() && f < (size_t)kBatchFinalists; f++) {
+        parent_tune_entry& te = e->last_tune[rank[f]];
+        double est = 0;
+        if (int rc = time_queue_sample(e, b, members, te.method, te.variant, &est)) return rc;
+        te.ms = est;
+        te.updates_per_s = total_upd / (est * 1e-3);
+        if (best < 0 || est < best) {
+            best = est;
+            *method = te.method;
+            *variant = te.variant;
+        }
+    }
     *pred_ms = best;
     return PARENT_OK;
 }
@@ -882,8 +1038,8 @@ constexpr int kShortFramesPerCell = 20;
 // much of their time on the fixed part (clear, epilogue), and their counts
 // stay small enough for narrower counters, so the two classes favour
 // different configurations.  The split is kept when the two classes'
-// predicted times add up to less than 0.99 of the merged queue's (classes 1),
-// always (2) or never (0).  e->last_tune keeps the merged queue's table.
+// estimated times add up to less than the merged queue's (classes 1), always
+// (2) or never (0).  e->last_tune keeps the merged queue's table.
 int plan_batch(parent_engine* e, parent_batch* b) {
     const int n = (int)b->items.size();
     std::vector<int> all(n);
@@ -902,58 +1058,11 @@ int plan_batch(parent_engine* e, parent_batch* b) {
         double p_long = 0, p_short = 0;
         if (int rc = batch_autotune(e, b, lng.members, &lng.method, &lng.variant, &p_long)) return rc;
         if (int rc = batch_autotune(e, b, shrt.members, &shrt.method, &shrt.variant, &p_short)) return rc;
-        if (b->classes == 2 || p_long + p_short < 0.99 * p_all) plan = {lng, shrt};
+        if (b->classes == 2 || p_long + p_short < p_all) plan = {lng, shrt};
     }
     e->last_tune = table;
     b->tuned_plan = plan;
     b->tuned_items = n;
-    return PARENT_OK;
-}
-
-// Uploads the merged tile queue of one launch: its items by decreasing frame
-// count (cost per tile), each item's tiles in its band order, padded with
-// empty tiles to a multiple of 8 and dealt so that block b of a launch runs
-// entry b of this list whatever XCD it lands on (the kernels map block b to
-// list slot (b % 8) * chunk + b / 8).
-int build_queue(parent_engine* e, parent_batch* b, const BinSet& env, parent_batch::Queue& qu) {
-    const parent_batch::Plan& pl = qu.plan;
-    int tr, tc;
-    tile_shape(env, pl.method, pl.variant, &tr, &tc);
-    std::vector<int> order = pl.members;
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return b->items[x]->traj.N > b->items[y]->traj.N; });
-    std::vector<int> lst;
-    for (int k : order) {
-        const Traj& t = b->items[k]->traj;
-        std::vector<int> tl = make_tiles(t.D, t.Np, tr, tc);
-        for (size_t i = 0; i + 1 < tl.size(); i += 2) {
-            lst.push_back(tl[i]);
-            lst.push_back(tl[i + 1]);
-            lst.push_back(k);
-        }
-    }
-    while ((lst.size() / 3) % 8) {
-        const Traj& t0 = b->items[order[0]]->traj;
-        lst.push_back(t0.D);  // no valid pair: r0 = c0 = D
-        lst.push_back(t0.D);
-        lst.push_back(order[0]);
-    }
-    if (lst.size() / 3 > (size_t)INT32_MAX) return bfail(b, PARENT_E_UNSUPPORTED, "batch tile queue too long");
-    const int T = (int)(lst.size() / 3);
-    // the launchers issue the queue in launches of `seg` tiles (a multiple
-    // of 8), each with its own block-to-slot map: deal each segment alone
-    const int seg = pl.method == PARENT_METHOD_MFMA ? mfma_tiles_per_launch() : atomic_tiles_per_launch(pl.variant);
-    std::vector<int> q(lst.size());
-    for (int s0 = 0; s0 < T; s0 += seg) {
-        const int n_seg = std::min(seg, T - s0), chunk = n_seg / 8;
-        for (int blk = 0; blk < n_seg; blk++) {
-            const int slot = s0 + (blk % 8) * chunk + blk / 8;
-            for (int c = 0; c < 3; c++) q[3 * (size_t)slot + c] = lst[3 * ((size_t)s0 + blk) + c];
-        }
-    }
-    HIPCHK(e, qu.tiles.ensure(q.size() * 4));
-    HIPCHK(e, hipMemcpyAsync(qu.tiles.p, q.data(), q.size() * 4, hipMemcpyHostToDevice, e->stream));
-    HIPCHK(e, hipStreamSynchronize(e->stream));
-    qu.ntiles = T;
     return PARENT_OK;
 }
 
@@ -1618,15 +1727,7 @@ int parent_batch_run(parent_batch* b, int method) {
         }
         plan = {one};
     }
-    if (b->descs_items != n || b->descs_nlogn != e->scratch.nlogn.p) {
-        std::vector<PairInputs> d(n);
-        for (int k = 0; k < n; k++) d[k] = with_nlogn(e, pair_inputs(b->items[k]->traj, b->items[k]->set));
-        HIPCHK(e, b->descs.ensure(d.size() * sizeof(PairInputs)));
-        HIPCHK(e, hipMemcpyAsync(b->descs.p, d.data(), d.size() * sizeof(PairInputs), hipMemcpyHostToDevice, e->stream));
-        HIPCHK(e, hipStreamSynchronize(e->stream));
-        b->descs_items = n;
-        b->descs_nlogn = e->scratch.nlogn.p;
-    }
+    if (int rc = ensure_descs(e, b)) return rc;
     if (b->queues.size() != plan.size()) {
         b->queues.clear();
         for (size_t i = 0; i < plan.size(); i++) b->queues.emplace_back(new parent_batch::Queue());
