@@ -1081,6 +1081,12 @@ namespace {
 // and 6: the layout of kind 4 in k_pairs_multi, 2 consecutive tiles per
 // workgroup (5) or 1 (6); kind 7: k_pairs_multi with 4 sub-counters per
 // cell, slot size in groups of 4 words (5050 -> B <= 100, 2887 -> B <= 75).
+// A CU holds 32 waves: 1,024-thread workgroups of the 820-group layout
+// (52.5 KB) run two to a CU, 512-thread ones three (variants 59, 60: 40 bins
+// 150.7 -> 135.3 ms on 4,494 dofs x 100k frames).  Block sizes stay
+// multiples of 256, one wave per SIMD each: 384 and 640 threads left SIMDs
+// unevenly loaded and ran slower (128.7 against 126.1 ms at 30 bins, 152.5
+// against 135.5 ms at 40).
 #define PARENT_ATOMIC_VARIANTS(X)     \
     X(0, 0, 1, 1, 256, 2560)          \
     X(1, 0, 2, 2, 256, 2560)          \
@@ -1140,7 +1146,9 @@ namespace {
     X(55, 6, 1, 1, 256, 465)          \
     X(56, 7, 1, 1, 1024, 5050)        \
     X(57, 7, 1, 1, 1024, 2887)        \
-    X(58, 7, 1, 1, 512, 2887)
+    X(58, 7, 1, 1, 512, 2887)         \
+    X(59, 6, 1, 1, 512, 820)          \
+    X(60, 5, 1, 1, 512, 820)
 
 struct AtomicVariant {
     AtomicConfig cfg;
