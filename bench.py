@@ -53,6 +53,7 @@ WORKLOADS = {
 SWEEP_BINS = [5, 10, 20, 30, 40, 50, 75, 100]
 BATCH_N, BATCH_BINS, BATCH_SEED = 64, 30, 4
 INT8_PEAK_OPS = 5.0e15  # MI355X dense int8 MFMA
+FP4_PEAK_OPS = 10.1e15  # MI355X dense FP4 MFMA
 HBM_PEAK = 8.0e12
 LDS_PEAK_FALLBACK = {"conflict_free": 9.5e12, "random_2500": 5.2e12}  # lds_peak.x on MI355X
 # HBM bytes per pass of the pair kernel, from rocprofv3 FETCH_SIZE and WRITE_SIZE passes scaled
@@ -191,9 +192,11 @@ def roofline_of(pa, method, variant, bins, updates, kernel_ms, workload, D, P, f
     rate = updates / (kernel_ms / 1e3)
     mfma_ceiling = INT8_PEAK_OPS / (2.0 * bins * bins)
     if method == pa.METHOD_MFMA:
-        r = {"bound": "mfma", "achieved": 2.0 * bins * bins * rate / 1e12, "peak": INT8_PEAK_OPS / 1e12,
-             "unit": "TFLOP/s", "frac": 2.0 * bins * bins * rate / INT8_PEAK_OPS, "traffic": None,
-             "algorithmic_work_per_launch": f"{2.0 * bins * bins * updates:.4e} int8 ops (2 B^2 per update)"}
+        # variant 1 multiplies FP4 operands: its peak is the dense FP4 rate
+        peak, kind = (FP4_PEAK_OPS, "fp4") if variant == 1 else (INT8_PEAK_OPS, "int8")
+        r = {"bound": "mfma", "achieved": 2.0 * bins * bins * rate / 1e12, "peak": peak / 1e12,
+             "unit": "TFLOP/s", "frac": 2.0 * bins * bins * rate / peak, "traffic": None,
+             "algorithmic_work_per_launch": f"{2.0 * bins * bins * updates:.4e} {kind} ops (2 B^2 per update)"}
     else:
         peak = lds_peak()
         r = {"bound": "lds_atomic", "achieved": rate / 1e9, "peak": peak["conflict_free"] / 1e9,
@@ -239,7 +242,7 @@ def run_single(args, pa, nthreads):
     del vals
     b6 = [bins] * 6
     if args.variant >= 0:
-        eng.set_variant(pa.METHOD_ATOMIC, args.variant)
+        eng.set_variant(pa.METHOD_MFMA if args.method == "mfma" else pa.METHOD_ATOMIC, args.variant)
 
     def step(m):
         eng.prepare(b6)
@@ -503,7 +506,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mist", action="store_true")
     ap.add_argument("--tune-report", action="store_true", help="print the autotuner's table")
-    ap.add_argument("--variant", type=int, default=-1, help="pin the atomic method's configuration")
+    ap.add_argument("--variant", type=int, default=-1,
+                    help="pin the atomic method's configuration (with --method mfma: 0 int8, 1 FP4)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this benchmark runs on one GPU only (--gpus {args.gpus} requested); "

@@ -297,8 +297,11 @@ int parent_engine_autotune(parent_engine* eng, parent_tune_entry* entries, int m
  * variant whose histograms do not fit the bins makes the method fall back to
  * PARENT_METHOD_PLAIN.  For PARENT_METHOD_SLABS the variant caps the
  * histogram rows per slab (0 or -1: as many as the LDS holds), which
- * exercises several slabs at small bin counts.  For testing every
- * configuration the autotuner may pick; no counterpart in the reference. */
+ * exercises several slabs at small bin counts.  For PARENT_METHOD_MFMA,
+ * variant 0 multiplies int8 one-hot operands and variant 1 FP4 ones (FP32
+ * sums; 8 to 12 bins and fewer than 2^24 frames, else variant 0 runs).  For
+ * testing every configuration the autotuner may pick; no counterpart in the
+ * reference. */
 int parent_engine_set_variant(parent_engine* eng, int method, int variant);
 /* How parent_engine_mist finds the tree: 0 (default) the spanning-tree path
  * where it applies (dense MI matrix fits, V <= 16,384, no equal candidates
@@ -364,8 +367,8 @@ int parent_batch_run(parent_batch* b, int method);
  * counterpart in the reference (it runs one trajectory per process). */
 int parent_batch_set_classes(parent_batch* b, int mode);
 /* parent_engine_set_variant for the batch's kernels (method
- * PARENT_METHOD_ATOMIC or PARENT_METHOD_SLABS): pins the configuration that
- * parent_batch_run(b, PARENT_METHOD_ATOMIC) uses.  For testing; no
+ * PARENT_METHOD_ATOMIC, PARENT_METHOD_MFMA or PARENT_METHOD_SLABS): pins the
+ * configuration that parent_batch_run(b, method) uses.  For testing; no
  * counterpart in the reference. */
 int parent_batch_set_variant(parent_batch* b, int method, int variant);
 

@@ -473,7 +473,8 @@ class Batch:
         return idx.value
 
     def set_variant(self, method: int, variant: int) -> None:
-        """Pins the configuration `run(METHOD_ATOMIC)` uses (-1: default)."""
+        """Pins the configuration `run(method)` uses for METHOD_ATOMIC,
+        METHOD_MFMA (0 int8, 1 FP4) or METHOD_SLABS (-1: default)."""
         self._check(lib().parent_batch_set_variant(self._h, method, variant))
 
     def set_classes(self, mode: int) -> None:

@@ -137,6 +137,7 @@ struct parent_engine {
     std::vector<Tuned> tuned;
     std::vector<parent_tune_entry> last_tune;
     int forced_atomic = -1;  // parent_engine_set_variant
+    int forced_mfma = -1;    // parent_engine_set_variant(PARENT_METHOD_MFMA, ...)
     int forced_slab_rows = 0;
     int mist_path_mode = 0;    // parent_engine_set_mist_path  // idem for PARENT_METHOD_SLABS: rows per slab (0: as many as the LDS holds)
 
@@ -424,7 +425,7 @@ PairInputs with_nlogn(parent_engine* e, PairInputs in) {
 // one launch of a tiled method's variant over a tile queue
 hipError_t launch_tiled(parent_engine* e, const PairInputs& in, int bmax, int method, int id, const TileQueue& q,
                         const int* db, const int* de, int* dout) {
-    if (method == PARENT_METHOD_MFMA) return launch_pairs_mfma(in, bmax, q, db, de, dout, e->stream);
+    if (method == PARENT_METHOD_MFMA) return launch_pairs_mfma(in, bmax, id, q, db, de, dout, e->stream);
     return launch_pairs_atomic(in, id, q, db, de, dout, e->stream);
 }
 
@@ -610,7 +611,10 @@ int resolve_method(parent_engine* e, const Traj& tj, BinSet& s, int method, int*
     }
     if (method == PARENT_METHOD_AUTO) return tuned_choice(e, tj, s, used, variant);
     if (method == PARENT_METHOD_MFMA) {
-        if (mfma_fits(e, s)) return PARENT_OK;
+        if (mfma_fits(e, s)) {
+            *variant = e->forced_mfma >= 0 && mfma_variant_fits(e->forced_mfma, bmax2d(s), tj.N) ? e->forced_mfma : 0;
+            return PARENT_OK;
+        }
         *used = PARENT_METHOD_ATOMIC;
     }
     if (*used == PARENT_METHOD_ATOMIC) {
@@ -1262,6 +1266,7 @@ int parent_engine_sweep(parent_engine* e, int n_sets, const int* bins, int metho
     key.push_back(method);
     key.push_back((int)e->traj_serial);
     key.push_back(e->forced_atomic);
+    key.push_back(e->forced_mfma);
     if (key != e->graph_key) drop_graph(e);
     while ((int)e->sets.size() < n_sets) e->sets.emplace_back(new BinSet());
     e->sets.resize(n_sets);
@@ -1525,7 +1530,14 @@ int parent_engine_set_variant(parent_engine* e, int method, int variant) {
         drop_graph(e);
         return PARENT_OK;
     }
-    if (method != PARENT_METHOD_ATOMIC) return fail(e, PARENT_E_INVALID, "only PARENT_METHOD_ATOMIC and PARENT_METHOD_SLABS have variants");
+    if (method == PARENT_METHOD_MFMA) {
+        if (variant < -1 || variant > 1) return fail(e, PARENT_E_INVALID, "MFMA variant must be -1, 0 or 1");
+        e->forced_mfma = variant;
+        drop_graph(e);
+        return PARENT_OK;
+    }
+    if (method != PARENT_METHOD_ATOMIC)
+        return fail(e, PARENT_E_INVALID, "only PARENT_METHOD_ATOMIC, PARENT_METHOD_MFMA and PARENT_METHOD_SLABS have variants");
     if (variant < -1 || variant >= atomic_config_count()) return fail(e, PARENT_E_INVALID, "variant out of range");
     e->forced_atomic = variant;
     return PARENT_OK;
@@ -1556,8 +1568,8 @@ int parent_engine_set_mist_path(parent_engine* e, int mode) {
 int parent_engine_variant_count(parent_engine* e, int method, int* count) {
     if (!e || !count) return PARENT_E_INVALID;
     *count = method == PARENT_METHOD_ATOMIC ? atomic_config_count()
-                                            : (method == PARENT_METHOD_PLAIN || method == PARENT_METHOD_MFMA ||
-                                               method == PARENT_METHOD_SLABS ? 1 : 0);
+             : method == PARENT_METHOD_MFMA   ? 2
+                                              : (method == PARENT_METHOD_PLAIN || method == PARENT_METHOD_SLABS ? 1 : 0);
     return PARENT_OK;
 }
 
@@ -1716,7 +1728,7 @@ int parent_batch_run(parent_batch* b, int method) {
         std::iota(one.members.begin(), one.members.end(), 0);
         if (method == PARENT_METHOD_MFMA && mfma_fits(e, env)) {
             one.method = PARENT_METHOD_MFMA;
-            one.variant = 0;
+            one.variant = e->forced_mfma >= 0 && mfma_variant_fits(e->forced_mfma, bmax2d(env), nmax) ? e->forced_mfma : 0;
         } else {
             // ATOMIC, MFMA outside its bin range, and PLAIN (which has no tile
             // queue): the tiled atomic kernel, in a configuration whose counters

@@ -117,7 +117,13 @@ int mfma_tiles_per_launch();
 // (0 when the 2D bin count is outside the kernel's range)
 int mfma_tile_dofs(int bmax);
 size_t mfma_lds_bytes();
-hipError_t launch_pairs_mfma(const PairInputs& in, int bmax, const TileQueue& q, const int* dump_begin,
+// variant 0: int8 operands (v_mfma_i32_32x32x32_i8); variant 1: FP4 one-hot
+// operands (v_mfma_scale_f32_32x32x64_f8f6f4, FP32 sums), 8 to 12 bins and
+// below 2^24 frames (mfma_variant_fits).  The autotuners offer variant 0
+// only: variant 1 ran within -2% to +5% of it (faster at 8 bins only), and
+// the tuning samples did not tell the two apart reliably.
+bool mfma_variant_fits(int variant, int bmax, int n_frames);
+hipError_t launch_pairs_mfma(const PairInputs& in, int bmax, int variant, const TileQueue& q, const int* dump_begin,
                              const int* dump_entries, int* dump_out, hipStream_t s);
 
 // device Prim (MIST)

@@ -21,22 +21,38 @@
 //     16g + e sits in k slot 32(e/4) + 4g + e%4, so the 8 frame groups of one
 //     dof write 8 consecutive LDS words in each scatter instruction.
 // Padding frames carry bin 255 and set nothing.  The accumulators then go to
-// an LDS count tile and each wave reduces whole pairs with the fused entropy
-// epilogue of the atomic kernels.
+// an LDS count tile, and one thread (up to 12 bins) or one wave reduces each
+// pair with the fused entropy epilogue of the atomic kernels.
+//
+// Variant 1 (8 to 12 bins, fewer than 2^24 frames) builds FP4 operands for
+// v_mfma_scale_f32_32x32x64_f8f6f4 instead: twice the k slots per
+// instruction, one-hot nibbles (1.0 = 0b0010, unit E8M0 scales) and FP32
+// accumulators, whose sums of 0/1 products are exact integers below 2^24.
 //
 // Lane layouts used (gfx950 32x32x32 int8): A operand lane l = row l % 32,
 // 16 bytes = k slots 16*(l/32) .. +15; B operand lane l = column l % 32, the
 // same k slots; result value v of lane l = row (v/4)*8 + (l/32)*4 + v%4,
-// column l % 32.  Any bijection of k slots onto frames gives the same sums as
-// long as A and B share it.
+// column l % 32.  The FP4 form: 32 k slots a lane, 32*(l/32) .. +31, in its
+// first four operand registers; the same result layout.  Any bijection of k
+// slots onto frames gives the same sums as long as A and B share it.
 #include "kernels.h"
+
+#include <type_traits>
 
 namespace parent {
 
 namespace {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v8i __attribute__((ext_vector_type(8)));
 typedef int v16i __attribute__((ext_vector_type(16)));
+typedef float v16f __attribute__((ext_vector_type(16)));
+
+// FP4 operands (e2m1) of v_mfma_scale_f32_32x32x64_f8f6f4: a one-hot value
+// is 1.0 = 0b0010 in its nibble; the E8M0 scale 127 = 2^0 leaves products
+// at 0 or 1, and FP32 sums of them are exact integers below 2^24.
+constexpr unsigned kUnitScales = 0x7F7F7F7Fu;
+constexpr int kFp4Format = 4;  // cbsz / blgp operand format code of FP4 (e2m1)
 
 constexpr int kTile = 128;        // output rows / columns per workgroup
 constexpr int kChunk = 128;       // frames per LDS operand chunk
@@ -100,9 +116,14 @@ __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(
 // per thread, ceil(2 * TD * 8 / 256); every thread issues all UPT loads, so
 // the waits for a staged chunk count the loads in flight exactly.  PERM
 // kernels with UPT < 4 fit 128 VGPRs and 36 KB of LDS: four workgroups per
-// CU (UPT 4, at 2 and 3 bins, three: it would spill at 128).
-template <bool BATCH, int PERM, int UPT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PERM ? (UPT < 4 ? 4 : 3) : 2))) void k_pairs_mfma(PairInputs in0, int B, int TD, TileQueue q,
+// CU (UPT 4, at 2 and 3 bins, three: it would spill at 128).  FP4: the
+// operands of v_mfma_scale_f32_32x32x64_f8f6f4 (64 frames a k-step, one-hot
+// nibbles 0b0010 = 1.0, FP32 sums), built with two v_perm_b32 and a
+// v_lshl_or_b32 per 8 frames; these kernels need 162 VGPRs and run three
+// workgroups per CU (at 128 the accumulators went to scratch), for one
+// staging unit per thread (8 to 12 bins) only.
+template <bool BATCH, int PERM, int UPT, bool FP4 = false>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PERM ? (UPT < 4 && !FP4 ? 4 : 3) : 2))) void k_pairs_mfma(PairInputs in0, int B, int TD, TileQueue q,
                                                        int t_begin, int t_end, const int* __restrict__ dump_begin,
                                                        const int* __restrict__ dump_entries, int* __restrict__ dump_out) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
@@ -118,7 +139,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PERM ? (UPT
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int wr = wid >> 1, wc = wid & 1;
 
-    v16i acc[2][2];
+    static_assert(!FP4 || PERM != 0, "FP4 operands are register-built");
+    typedef typename std::conditional<FP4, v16f, v16i>::type acc_t;
+    acc_t acc[2][2];
 #pragma unroll
     for (int i = 0; i < 2; i++)
 #pragma unroll
@@ -191,9 +214,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PERM ? (UPT
                 const int row = (s2 ? wc : wr) * 64 + i * 32 + (lane & 31);
                 const bool ok = row < TD * B;
                 const uint2 tab = onehot_table(ok, row % B);
-                tlo[s2][i] = tab.x;
-                thi[s2][i] = tab.y;
-                toff[s2][i] = s2 * side_bytes + (ok ? row / B : 0) * kIdxStride + (lane >> 5) * 16 +
+                // (FP4: table bytes 0x02, the e2m1 1.0 in a low nibble)
+                tlo[s2][i] = FP4 ? tab.x << 1 : tab.x;
+                thi[s2][i] = FP4 ? tab.y << 1 : tab.y;
+                toff[s2][i] = s2 * side_bytes + (ok ? row / B : 0) * kIdxStride + (lane >> 5) * (FP4 ? 32 : 16) +
                               (ok && row % B >= 8 ? alt : 0);
             }
         auto onehot = [&](int s2, int i, const uint4& x) {
@@ -202,6 +226,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PERM ? (UPT
 #pragma unroll
             for (int e = 0; e < 4; e++) {
                 r[e] = (int)__builtin_amdgcn_perm(thi[s2][i], tlo[s2][i], words[e]);
+            }
+            return r;
+        };
+        // FP4: the lane's 32 frames of a 64-frame k-step, two words of 4
+        // frames per operand register: byte b of register v holds frame
+        // 8v + b in its low nibble and frame 8v + 4 + b in its high one (a
+        // bijection of k slots onto frames, the same for A and B).
+        auto onehot4 = [&](int s2, int i, const uint4& x0, const uint4& x1) {
+            const unsigned w[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+            v8i r;
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const unsigned lo = __builtin_amdgcn_perm(thi[s2][i], tlo[s2][i], w[2 * v]);
+                const unsigned hi = __builtin_amdgcn_perm(thi[s2][i], tlo[s2][i], w[2 * v + 1]);
+                r[v] = (int)(lo | (hi << 4));
+                r[v + 4] = 0;
             }
             return r;
         };
@@ -227,17 +267,41 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PERM ? (UPT
                 // (Issuing all 16 operand reads of the chunk first, pinned by a
                 // scheduling barrier, needs 64 more VGPRs: at three waves per
                 // SIMD it was 4% slower than this at four.)
+                if constexpr (FP4) {
+                    // B operands first, then one A operand at a time, each
+                    // group fenced off from the next
+                    auto build = [&](int s2, int i, int ks) {
+                        const int off = toff[s2][i] + ks * 64;
+                        const uint4* p = reinterpret_cast<const uint4*>(st + off);
+                        return onehot4(s2, i, p[0], p[1]);
+                    };
 #pragma unroll
-                for (int ks = 0; ks < kChunk / 32; ks++) {
-                    v4i a[2], b[2];
+                    for (int ks = 0; ks < kChunk / 64; ks++) {
+                        const v8i b0 = build(1, 0, ks), b1 = build(1, 1, ks);
+                        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-                    for (int i = 0; i < 2; i++) a[i] = onehot(0, i, *reinterpret_cast<const uint4*>(st + toff[0][i] + ks * 32));
+                        for (int i = 0; i < 2; i++) {
+                            const v8i a = build(0, i, ks);
+                            acc[i][0] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
+                                a, b0, acc[i][0], kFp4Format, kFp4Format, 0, kUnitScales, 0, kUnitScales);
+                            acc[i][1] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
+                                a, b1, acc[i][1], kFp4Format, kFp4Format, 0, kUnitScales, 0, kUnitScales);
+                            __builtin_amdgcn_sched_barrier(0);
+                        }
+                    }
+                } else {
 #pragma unroll
-                    for (int j = 0; j < 2; j++) b[j] = onehot(1, j, *reinterpret_cast<const uint4*>(st + toff[1][j] + ks * 32));
+                    for (int ks = 0; ks < kChunk / 32; ks++) {
+                        v4i a[2], b[2];
 #pragma unroll
-                    for (int i = 0; i < 2; i++)
+                        for (int i = 0; i < 2; i++) a[i] = onehot(0, i, *reinterpret_cast<const uint4*>(st + toff[0][i] + ks * 32));
 #pragma unroll
-                        for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[i], b[j], acc[i][j], 0, 0, 0);
+                        for (int j = 0; j < 2; j++) b[j] = onehot(1, j, *reinterpret_cast<const uint4*>(st + toff[1][j] + ks * 32));
+#pragma unroll
+                        for (int i = 0; i < 2; i++)
+#pragma unroll
+                            for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[i], b[j], acc[i][j], 0, 0, 0);
+                    }
                 }
                 if (c + 1 < nch) stage((c + 1) & 1, next, c + 1);
             }
@@ -361,7 +425,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PERM ? (UPT
                 for (int v = 0; v < 16; v++) {
                     const int row = wr * 64 + i * 32 + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
                     const int col = wc * 64 + j * 32 + (lane & 31);
-                    if (row >= rlo && row < rhi) cnt[(row - rlo) * kTile + col] = acc[i][j][v];
+                    if (row >= rlo && row < rhi) cnt[(row - rlo) * kTile + col] = (int)acc[i][j][v];
                 }
         __syncthreads();
 
@@ -420,9 +484,18 @@ int mfma_tile_dofs(int bins) {
 
 size_t mfma_lds_bytes() { return kLds; }
 
+// FP4 sums are FP32: exact while every count stays below 2^24.  The FP4
+// kernels exist for one staging unit per thread (8 to 12 bins): with two,
+// their operand builds spill even at three waves per SIMD.
+bool mfma_variant_fits(int variant, int bmax, int n_frames) {
+    if (mfma_tile_dofs(bmax) < 1) return false;
+    if (variant == 0) return true;
+    return variant == 1 && bmax >= 8 && bmax <= kPermMaxBins && n_frames < (1 << 24);
+}
+
 int mfma_tiles_per_launch() { return tiles_per_launch(256); }
 
-hipError_t launch_pairs_mfma(const PairInputs& in, int bins, const TileQueue& q, const int* dump_begin,
+hipError_t launch_pairs_mfma(const PairInputs& in, int bins, int variant, const TileQueue& q, const int* dump_begin,
                              const int* dump_entries, int* dump_out, hipStream_t s) {
     const int ntiles = q.ntiles;
     const int TD = mfma_tile_dofs(bins);
@@ -430,46 +503,50 @@ hipError_t launch_pairs_mfma(const PairInputs& in, int bins, const TileQueue& q,
     const int perm = bins <= 8 ? 1 : bins <= kPermMaxBins ? 2 : 0;
     const int upt = (2 * TD * 8 + 255) / 256;  // 1, 2, 3 or 4; 3 runs as 4
     const bool batch = q.descs != nullptr;
+    // variant 1: FP4 operands, register-built kernels of one staging unit per thread
+    const bool fp4 = variant == 1;
+    if (variant < 0 || variant > 1 || (fp4 && (perm == 0 || upt != 1))) return hipErrorInvalidValue;
     // kernel index: 0 scatter, 1-3 PERM 1 with 1, 2, 4 units per thread, 4 PERM 2 (TD <= 14: 1 unit)
     const int kid = perm == 0 ? 0 : perm == 2 ? 4 : upt == 1 ? 1 : upt == 2 ? 2 : 3;
-    const void* fns[2][5] = {{reinterpret_cast<const void*>(&k_pairs_mfma<false, 0, 4>),
-                              reinterpret_cast<const void*>(&k_pairs_mfma<false, 1, 1>),
-                              reinterpret_cast<const void*>(&k_pairs_mfma<false, 1, 2>),
-                              reinterpret_cast<const void*>(&k_pairs_mfma<false, 1, 4>),
-                              reinterpret_cast<const void*>(&k_pairs_mfma<false, 2, 1>)},
-                             {reinterpret_cast<const void*>(&k_pairs_mfma<true, 0, 4>),
-                              reinterpret_cast<const void*>(&k_pairs_mfma<true, 1, 1>),
-                              reinterpret_cast<const void*>(&k_pairs_mfma<true, 1, 2>),
-                              reinterpret_cast<const void*>(&k_pairs_mfma<true, 1, 4>),
-                              reinterpret_cast<const void*>(&k_pairs_mfma<true, 2, 1>)}};
+#define PARENT_MFMA_FN(BT, PM, U, F) reinterpret_cast<const void*>(&k_pairs_mfma<BT, PM, U, F>)
+    const void* fns[2][2][5] = {
+        {{PARENT_MFMA_FN(false, 0, 4, false), PARENT_MFMA_FN(false, 1, 1, false), PARENT_MFMA_FN(false, 1, 2, false),
+          PARENT_MFMA_FN(false, 1, 4, false), PARENT_MFMA_FN(false, 2, 1, false)},
+         {nullptr, PARENT_MFMA_FN(false, 1, 1, true), nullptr, nullptr, PARENT_MFMA_FN(false, 2, 1, true)}},
+        {{PARENT_MFMA_FN(true, 0, 4, false), PARENT_MFMA_FN(true, 1, 1, false), PARENT_MFMA_FN(true, 1, 2, false),
+          PARENT_MFMA_FN(true, 1, 4, false), PARENT_MFMA_FN(true, 2, 1, false)},
+         {nullptr, PARENT_MFMA_FN(true, 1, 1, true), nullptr, nullptr, PARENT_MFMA_FN(true, 2, 1, true)}}};
+#undef PARENT_MFMA_FN
     if (perm == 2 && upt != 1) return hipErrorInvalidValue;
     if (perm && (64 + bins - 1) / bins * bins > kPermCountRows) return hipErrorInvalidValue;
     const int lds = perm ? kPermLds : kLds;
-    hipError_t e = hipFuncSetAttribute(fns[batch][kid], hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    hipError_t e = hipFuncSetAttribute(fns[batch][fp4][kid], hipFuncAttributeMaxDynamicSharedMemorySize, lds);
     if (e != hipSuccess) return e;
     const int per_launch = mfma_tiles_per_launch();
     for (int t0 = 0; t0 < ntiles; t0 += per_launch) {
         const int t1 = t0 + per_launch < ntiles ? t0 + per_launch : ntiles;
         const dim3 grid((t1 - t0 + 7) / 8 * 8);
-#define PARENT_MFMA_LAUNCH(BT, PM, U) \
-    k_pairs_mfma<BT, PM, U><<<grid, 256, lds, s>>>(in, bins, TD, q, t0, t1, dump_begin, dump_entries, dump_out)
+#define PARENT_MFMA_LAUNCH(BT, PM, U, F) \
+    k_pairs_mfma<BT, PM, U, F><<<grid, 256, lds, s>>>(in, bins, TD, q, t0, t1, dump_begin, dump_entries, dump_out)
+#define PARENT_MFMA_SWITCH(BT)                                   \
+    if (fp4) {                                                   \
+        if (kid == 4) PARENT_MFMA_LAUNCH(BT, 2, 1, true);        \
+        else PARENT_MFMA_LAUNCH(BT, 1, 1, true);                 \
+    } else {                                                     \
+        switch (kid) {                                           \
+            case 1: PARENT_MFMA_LAUNCH(BT, 1, 1, false); break;  \
+            case 2: PARENT_MFMA_LAUNCH(BT, 1, 2, false); break;  \
+            case 3: PARENT_MFMA_LAUNCH(BT, 1, 4, false); break;  \
+            case 4: PARENT_MFMA_LAUNCH(BT, 2, 1, false); break;  \
+            default: PARENT_MFMA_LAUNCH(BT, 0, 4, false); break; \
+        }                                                        \
+    }
         if (batch) {
-            switch (kid) {
-                case 1: PARENT_MFMA_LAUNCH(true, 1, 1); break;
-                case 2: PARENT_MFMA_LAUNCH(true, 1, 2); break;
-                case 3: PARENT_MFMA_LAUNCH(true, 1, 4); break;
-                case 4: PARENT_MFMA_LAUNCH(true, 2, 1); break;
-                default: PARENT_MFMA_LAUNCH(true, 0, 4); break;
-            }
+            PARENT_MFMA_SWITCH(true)
         } else {
-            switch (kid) {
-                case 1: PARENT_MFMA_LAUNCH(false, 1, 1); break;
-                case 2: PARENT_MFMA_LAUNCH(false, 1, 2); break;
-                case 3: PARENT_MFMA_LAUNCH(false, 1, 4); break;
-                case 4: PARENT_MFMA_LAUNCH(false, 2, 1); break;
-                default: PARENT_MFMA_LAUNCH(false, 0, 4); break;
-            }
+            PARENT_MFMA_SWITCH(false)
         }
+#undef PARENT_MFMA_SWITCH
 #undef PARENT_MFMA_LAUNCH
     }
     return hipGetLastError();

@@ -164,6 +164,26 @@ def test_batch_frame_classes():
             _check_vs_oracle(nD, v, bins, s1c, s2c)
 
 
+def test_batch_mfma_fp4():
+    """The batch instantiation of MFMA variant 1 (FP4) on uneven
+    trajectories at 10 bins: S2D against the oracle."""
+    pa = _pa()
+    bins = [10] * 6
+    shapes = [(10, 1500, 1), (23, 4097, 2), (17, 800, 3), (9, 130, 4)]
+    trajs = [pa.synth_values(a, n, s) for a, n, s in shapes]
+    print("batch mfma fp4", flush=True)
+    with pa.Batch(bins) as b:
+        for v in trajs:
+            b.add(v)
+        b.set_variant(pa.METHOD_MFMA, 1)
+        b.run(pa.METHOD_MFMA)
+        st = b.stats()
+        assert st["method"] == pa.METHOD_MFMA and st["kernel_variant"] == 1
+        for k, v in enumerate(trajs):
+            s1, s2 = b.results(k)
+            _check_vs_oracle(v.shape[1] // 3 - 1, v, bins, s1, s2)
+
+
 @pytest.mark.parametrize("method", [1, 2, 3])  # atomic, MFMA, auto
 def test_batch_with_short_trajectories(method):
     """Trajectories of 1 to 17 frames beside longer ones in one launch: the

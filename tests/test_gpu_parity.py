@@ -312,6 +312,56 @@ def test_mfma_counts_and_entropies(engine, bins, atoms, frames):
         assert np.array_equal(got[pairs.index((a, b))].astype(np.int64), ho)
 
 
+FP4_CASES = [c for c in MFMA_CASES if 8 <= max(c[0][3:]) <= 12]
+
+
+@pytest.mark.parametrize("bins,atoms,frames", FP4_CASES)
+def test_mfma_fp4_counts_and_entropies(engine, bins, atoms, frames):
+    """MFMA variant 1 (FP4 one-hot operands, FP32 sums, 8 to 12 bins): every
+    pair's histogram bit-exact against the plain kernel and S2D within
+    tolerance of the oracle."""
+    pa = _pa()
+    print(f"mfma fp4 bins={bins} atoms={atoms} frames={frames}", flush=True)
+    vals, nD = _case(atoms, frames, 19)
+    engine.set_trajectory(vals)
+    engine.prepare(bins)
+    engine.set_variant(pa.METHOD_MFMA, 1)
+    try:
+        engine.pairs(pa.METHOD_MFMA)
+        st = engine.stats()
+        assert st["method"] == pa.METHOD_MFMA and st["kernel_variant"] == 1
+        s2 = engine.s2d()
+        D = 3 * nD + 3
+        gi, gj = np.triu_indices(D, 1)
+        pairs = list(zip(gi.tolist(), gj.tolist()))
+        got = engine.pair_counts(pairs, pa.METHOD_MFMA, bins[3:])
+    finally:
+        engine.set_variant(pa.METHOD_MFMA, -1)
+    plain = engine.pair_counts(pairs, pa.METHOD_PLAIN, bins[3:])
+    for p, h, hp in zip(pairs, got, plain):
+        assert np.array_equal(h, hp), f"counts differ for pair {p}"
+    _, s2o, *_ = _oracle_full(vals, nD, bins)
+    assert np.max(np.abs(s2 - s2o)) <= S_ABS
+
+
+def test_mfma_fp4_outside_its_bins_runs_int8(engine):
+    """Variant 1 pinned at 5 bins (two staging units per thread, no FP4
+    kernel): the int8 kernel runs instead, with the same counts."""
+    pa = _pa()
+    vals, nD = _case(10, 1000, 23)
+    engine.set_trajectory(vals)
+    engine.prepare([5] * 6)
+    engine.set_variant(pa.METHOD_MFMA, 1)
+    try:
+        engine.pairs(pa.METHOD_MFMA)
+        st = engine.stats()
+    finally:
+        engine.set_variant(pa.METHOD_MFMA, -1)
+    assert st["method"] == pa.METHOD_MFMA and st["kernel_variant"] == 0
+    _, s2o, *_ = _oracle_full(vals, nD, [5] * 6)
+    assert np.max(np.abs(engine.s2d() - s2o)) <= S_ABS
+
+
 def test_mfma_out_of_range_falls_back(engine):
     """Above 64 bins the MFMA request runs on the atomic kernels."""
     pa = _pa()
