@@ -1086,7 +1086,11 @@ namespace {
 // 150.7 -> 135.3 ms on 4,494 dofs x 100k frames).  Block sizes stay
 // multiples of 256, one wave per SIMD each: 384 and 640 threads left SIMDs
 // unevenly loaded and ran slower (128.7 against 126.1 ms at 30 bins, 152.5
-// against 135.5 ms at 40).
+// against 135.5 ms at 40).  Variants 61-64: 4 sub-counters per cell sized
+// for 50, 30, 40 and 20 bins (20.4, 7.4, 13.1 and 3.4 KB): for short
+// trajectories, whose tiles spend much of their time clearing and reading a
+// 16-copy histogram (the 28 trajectories of under 18,000 frames of a
+// 64-trajectory batch at 30 bins: 119.6 against 152.4 ms).
 #define PARENT_ATOMIC_VARIANTS(X)     \
     X(0, 0, 1, 1, 256, 2560)          \
     X(1, 0, 2, 2, 256, 2560)          \
@@ -1148,7 +1152,11 @@ namespace {
     X(57, 7, 1, 1, 1024, 2887)        \
     X(58, 7, 1, 1, 512, 2887)         \
     X(59, 6, 1, 1, 512, 820)          \
-    X(60, 5, 1, 1, 512, 820)
+    X(60, 5, 1, 1, 512, 820)          \
+    X(61, 7, 1, 1, 256, 1275)         \
+    X(62, 7, 1, 1, 256, 465)          \
+    X(63, 7, 1, 1, 256, 820)          \
+    X(64, 7, 1, 1, 256, 210)
 
 struct AtomicVariant {
     AtomicConfig cfg;
