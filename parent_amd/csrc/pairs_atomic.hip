@@ -1089,7 +1089,7 @@ namespace {
 // against 135.5 ms at 40).  Variants 61-64: 4 sub-counters per cell sized
 // for 50, 30, 40 and 20 bins (20.4, 7.4, 13.1 and 3.4 KB): for short
 // trajectories, whose tiles spend much of their time clearing and reading a
-// 16-copy histogram (the 28 trajectories of under 18,000 frames of a
+// 16-copy histogram (the 25 trajectories of under 18,000 frames of a
 // 64-trajectory batch at 30 bins: 119.6 against 152.4 ms).
 #define PARENT_ATOMIC_VARIANTS(X)     \
     X(0, 0, 1, 1, 256, 2560)          \
