@@ -13,6 +13,8 @@
 //            2D ones for some dof type.
 #include "kernels.h"
 
+#include <algorithm>
+
 namespace parent {
 
 __device__ __forceinline__ unsigned long long ord_enc(double x) {
@@ -29,13 +31,6 @@ __device__ __forceinline__ double modfit_shift(double x, double mf, double twopi
     const double s = x + mf;
     const int k = (int)(s / twopi);
     return s - twopi * (double)k;
-}
-
-__global__ void k_fill_u64(unsigned long long* p, size_t n, unsigned long long v) {
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
-}
-__global__ void k_fill_u32(unsigned* p, size_t n, unsigned v) {
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }
 
 // Block order of the frame-major scans (1D grid of 8 * ceil(nbx * nby / 8)
@@ -585,9 +580,28 @@ size_t dof_stats_lds(int nb2, int nh1, int nb1) { return (size_t)((nb2 + nh1 + 1
 // S_ij then comes out of the same sum, S_ij = ln N + c_i + c_j - 1/(2N)
 // - (1/N) sum (n ln n - 1/2 [n > 0]), with no separate count of occupied
 // cells.
+__device__ __forceinline__ double nlogn_entry(int n) { return n == 0 ? 0.0 : (double)n * log((double)n) - 0.5; }
+
 __global__ void k_nlogn(int N, double* tab) {
-    for (int n = blockIdx.x * blockDim.x + threadIdx.x; n <= N; n += gridDim.x * blockDim.x)
-        tab[n] = n == 0 ? 0.0 : (double)n * log((double)n) - 0.5;
+    for (int n = blockIdx.x * blockDim.x + threadIdx.x; n <= N; n += gridDim.x * blockDim.x) tab[n] = nlogn_entry(n);
+}
+
+// The set-up of a prepare in one launch (each was its own small kernel, a
+// few microseconds apiece on short trajectories): the flags word, and with
+// the ranges the extrema accumulators (min at all ones, max at 0) and the
+// modfit histograms, and the n ln n table when nlogn is given.
+__global__ void k_prepare_init(int* flags, unsigned long long* emin, unsigned long long* emax, int D,
+                               unsigned* mfh, size_t nmf, double* nlogn, int N) {
+    const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (size_t)gridDim.x * blockDim.x;
+    if (tid == 0) *flags = 0;
+    if (emin)
+        for (size_t i = tid; i < (size_t)D; i += stride) {
+            emin[i] = ~0ull;
+            emax[i] = 0ull;
+        }
+    for (size_t i = tid; i < nmf; i += stride) mfh[i] = 0u;
+    if (nlogn)
+        for (size_t n = tid; n <= (size_t)N; n += stride) nlogn[n] = nlogn_entry((int)n);
 }
 
 // ---------------------------------------------------------------------------
@@ -637,11 +651,15 @@ hipError_t launch_prepare(const PrepareArgs& a, hipStream_t s) {
     const int nDih = nD;
     const double pi = a.pi, twopi = 2 * pi;
     const double mf_bin = (2 * pi + 0.000000005) / kModfitBins;
-    k_fill_u32<<<1, 32, 0, s>>>((unsigned*)a.flags, 1, 0u);
+    {
+        const bool ranges = (a.stages & kPrepareRanges) != 0;
+        const size_t nmf = ranges ? (size_t)nDih * kModfitBins : 0;
+        double* tab = (a.stages & kPrepareNlogn) ? a.nlogn : nullptr;
+        const size_t work = std::max(std::max(ranges ? (size_t)D : 1, nmf), tab ? (size_t)N + 1 : 1);
+        const unsigned blocks = (unsigned)std::min<size_t>(256, (work + 255) / 256);
+        k_prepare_init<<<blocks, 256, 0, s>>>(a.flags, ranges ? a.emin : nullptr, a.emax, D, a.mfhist, nmf, tab, N);
+    }
     if (a.stages & kPrepareRanges) {
-        k_fill_u64<<<64, 256, 0, s>>>(a.emin, (size_t)D, ~0ull);
-        k_fill_u64<<<64, 256, 0, s>>>(a.emax, (size_t)D, 0ull);
-        if (nDih > 0) k_fill_u32<<<64, 256, 0, s>>>(a.mfhist, (size_t)nDih * kModfitBins, 0u);
         const int nbx1 = (D + 63) / 64, fpb = scan_frames_per_block(nbx1, N);
         const int nby1 = (N + fpb - 1) / fpb;
         const unsigned g1 = xcd_grid(nbx1, nby1);
@@ -683,7 +701,6 @@ hipError_t launch_prepare(const PrepareArgs& a, hipStream_t s) {
         else PARENT_STATS(uint16_t, uint16_t);
 #undef PARENT_STATS
     }
-    if (a.stages & kPrepareNlogn) k_nlogn<<<(N + 256) / 256, 256, 0, s>>>(N, a.nlogn);
     return hipGetLastError();
 }
 
