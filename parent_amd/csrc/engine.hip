@@ -350,10 +350,11 @@ void push_band(std::vector<int>& t, int D, int tr, int tc, int b0, int b1) {
 }
 
 int tile_band(int tr, int Np) {
-    // MFMA tiles (16 dofs a side and more): 8 tile rows a band, so the ~128
-    // workgroups of an XCD run about 8 x 16 tiles at once and read each
-    // chunk of a row or column dof's indices from L2 for all of them
-    if (tr >= 16) return 8 * tr;
+    // MFMA tiles (8 dofs a side and more, B <= 16; the atomic kernels' tiles
+    // have at most 4): 8 tile rows a band, so the ~128 workgroups of an XCD
+    // run about 8 x 16 tiles at once and read each chunk of a row or column
+    // dof's indices from L2 for all of them
+    if (tr >= 8) return 8 * tr;
     const int rows = std::min(32, std::max(8, (int)(3276800LL / std::max(Np, 1))));
     return std::max(tr, rows / tr * tr);
 }
