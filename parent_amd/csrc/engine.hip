@@ -905,6 +905,8 @@ int time_queue_sample(parent_engine* e, parent_batch* b, const std::vector<int>&
             const long long lo = std::max(s0, a), hi = std::min(s0 + L, z);
             if (lo >= hi) continue;
             if (tl.empty()) tl = make_tiles(t.D, t.Np, tr, tc);
+            if ((long long)tl.size() != 2 * cnt[i])
+                return bfail(b, PARENT_E_INVALID, "tile list and tile count disagree");
             for (long long x = lo; x < hi; x++) {
                 lst.push_back(tl[2 * (x - a)]);
                 lst.push_back(tl[2 * (x - a) + 1]);
